@@ -1,0 +1,245 @@
+/*
+ * fem_mi355x.h -- C ABI of the MI355X (gfx950) P1 FEM assembly + AMG-PCG library.
+ *
+ * Plain C: opaque handle, POD parameter struct, pointers + sizes, integer
+ * status codes.  No function in this library calls exit(); every failure is a
+ * non-zero status with a message in fem_last_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (SCIInstitute/SCI-Solver_FEM, paths relative to the repository root).
+ *
+ * Threading: calls on one handle must be serialised by the caller; distinct
+ * handles are independent.  All device work of a handle is queued on one HIP
+ * stream (fem_set_stream), and every call returns only after that work is done.
+ */
+#ifndef FEM_MI355X_H
+#define FEM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum fem_status {
+  FEM_OK = 0,
+  FEM_ERR_ARG = 1,          /* invalid argument / inconsistent sizes        */
+  FEM_ERR_IO = 2,           /* file missing or malformed                    */
+  FEM_ERR_HIP = 3,          /* HIP runtime error                            */
+  FEM_ERR_EMPTY = 4,        /* no matrix (reference: std::invalid_argument) */
+  FEM_ERR_NOT_CONVERGED = 5,/* solve ended at maxIters above tolerance      */
+  FEM_ERR_NUMERIC = 6,      /* non-positive diagonal, NaN, coarse not SPD   */
+  FEM_ERR_STATE = 7,        /* call out of order (e.g. grad before solve)   */
+  FEM_ERR_CAPACITY = 8      /* a kernel capacity limit was exceeded         */
+};
+
+typedef struct fem_handle fem_handle;
+
+/* ---- solver parameters -----------------------------------------------------
+ * Field names and defaults of the reference's public FEMSolver members,
+ * src/FEMSolver.h:40-61, defaults src/FEMSolver.cu:11-33.  Fields marked
+ * "accepted, no effect" are read by nothing in the reference either
+ * (src/core/cuda/amg_level.cu:51,106; src/core/cuda/cgcycle.cu:7;
+ * src/FEMSolver.cu:99-129).  Fields after `blockSize` are additions. */
+typedef struct fem_params {
+  int verbose;            /* verbose_                                        */
+  int maxLevels;          /* maxLevels_          100                          */
+  int maxIters;           /* maxIters_           100                          */
+  int preInnerIters;      /* preInnerIters_      5  pre-smoothing sweeps      */
+  int postInnerIters;     /* postInnerIters_     5  post-smoothing sweeps     */
+  int postRelaxes;        /* postRelaxes_        1  post sweeps x this count  */
+  int cycleIters;         /* cycleIters_         1  accepted, no effect       */
+  int dsType;             /* dsType_             0  accepted, no effect       */
+  int topSize;            /* topSize_            256 coarsest level < this    */
+  int randMisParameters;  /* randMisParameters_  90102 digits: min aggregate
+                             size (x/10000%10), coarse depth, fine MIS depth */
+  int partitionMaxSize;   /* partitionMaxSize_   512 accepted, no effect      */
+  int aggregatorType;     /* aggregatorType_     0  (only 0 is implemented)   */
+  int convergeType;       /* convergeType_       0  accepted, no effect       */
+  double tolerance;       /* tolerance_          1e-6 relative residual       */
+  int cycleType;          /* cycleType_          0  V only                    */
+  int solverType;         /* solverType_         0: one V-cycle, 1: PCG       */
+  double smootherWeight;  /* smootherWeight_     1.0                          */
+  double proOmega;        /* proOmega_           0.67 prolongator smoothing   */
+  int device;             /* device_             0                            */
+  int blockSize;          /* blockSize_          256 accepted, no effect      */
+  /* --- additions --- */
+  int precision;          /* 0: FP64 hierarchy, 1: FP32 hierarchy + FP64 PCG  */
+  int smootherType;       /* 0: Jacobi, weight smootherWeight*4/(3*rho(D^-1A))
+                             1: Chebyshev of degree pre/postInnerIters        */
+  int aggregationRule;    /* 0: first aggregated neighbour in index order (the
+                             reference kernel's behaviour), 1: most-adjacent
+                             aggregate, ties to the lowest aggregate id       */
+  int coarseSolve;        /* 0: FP64 Cholesky-based dense inverse on device   */
+  unsigned long long seed;/* seed of the randomized MIS (reference: time())  */
+} fem_params;
+
+/* Fill *p with the reference defaults (src/FEMSolver.cu:11-33). */
+void fem_params_default(fem_params* p);
+
+/* ---- construction ----------------------------------------------------------*/
+/* FEMSolver::FEMSolver(fname, isTetMesh, verbose), src/FEMSolver.cu:9-44:
+ * reads <base>.node/<base>.ele (is_tet != 0, TetMesh::read,
+ * src/core/cuda/tetmesh.cu:251-376) or an ASCII PLY/OFF triangle mesh
+ * (TriMesh::read, src/core/aggmis/cuda/TriMesh_io.cu:146-257), builds the P1
+ * pattern and assembles stiffness + mass on the GPU
+ * (FEMSolver::getMatrixFromMesh, src/FEMSolver.cu:141-171).  The element
+ * coefficient is the reference's material-label table
+ * (src/core/cuda/perform_element_loop_3D.cuh:585-610), mass coefficient 1. */
+int fem_create_from_files(const char* base, int is_tet, int verbose, fem_handle** out);
+
+/* Same pipeline for a mesh given as arrays (the generated workloads).
+ * dim = 3: conn is ne x 4 tets; dim = 2: conn is ne x 3 triangles.
+ * xyz is nv x 3 (z ignored in 2D).  kappa: per-element conductivity or NULL
+ * (all ones).  Operator K = sum_e kappa_e S_e + mass_coeff * M.
+ * flags bit 0: 2D mass with the signed Jacobian
+ * (src/core/cuda/perform_element_loop_2D.cuh:92). */
+int fem_create_from_arrays(int dim, int64_t nv, const double* xyz, int64_t ne,
+                           const int32_t* conn, const double* kappa,
+                           double mass_coeff, int flags, int verbose,
+                           fem_handle** out);
+
+/* FEMSolver::~FEMSolver, src/FEMSolver.cu:46-51. */
+void fem_destroy(fem_handle* h);
+
+/* Message of the last failure on h (or of the last failed create if h is NULL). */
+const char* fem_last_error(const fem_handle* h);
+
+/* Public solver fields (src/FEMSolver.h:40-61).  Changing a field that shapes
+ * the hierarchy makes the next solve rebuild it. */
+int fem_set_params(fem_handle* h, const fem_params* p);
+int fem_get_params(const fem_handle* h, fem_params* p);
+
+/* Queue all device work of h on `stream` (a hipStream_t; NULL = own stream). */
+int fem_set_stream(fem_handle* h, void* stream);
+
+/* ---- matrix access ---------------------------------------------------------*/
+/* FEMSolver::getMatrixRows, src/FEMSolver.cu:95-97. */
+int64_t fem_get_matrix_rows(const fem_handle* h);
+int64_t fem_get_matrix_nnz(const fem_handle* h);
+int64_t fem_get_num_elements(const fem_handle* h);
+
+/* The assembled A_h_ (src/FEMSolver.cu:170) as CSR in user vertex numbering,
+ * columns ascending.  Any output pointer may be NULL. */
+int fem_get_matrix_csr(const fem_handle* h, int64_t* row_ptr, int32_t* cols, double* vals);
+
+/* FEMSolver::readMatlabSparseMatrix, src/FEMSolver.cu:177-356: replace A by
+ * the file matrix plus 1e-12 on every mesh-pattern slot.  Returns 0, or the
+ * reference's 1 / -1 codes on a malformed file (message in fem_last_error). */
+int fem_read_matlab_sparse(fem_handle* h, const char* filename);
+
+/* Replace the per-element conductivity (user element order) and reassemble
+ * in place on the device; the hierarchy is rebuilt by the next setup/solve. */
+int fem_update_kappa(fem_handle* h, const double* kappa);
+
+/* ---- setup and solve -------------------------------------------------------*/
+/* AMG::setup, src/core/cuda/amg.cu:79-144: aggregation, smoothed prolongator,
+ * Galerkin coarse operators, smoother data, coarsest-level factorisation.
+ * Idempotent; fem_solve calls it when needed. */
+int fem_setup(fem_handle* h);
+
+typedef struct fem_solve_info {
+  int iterations;          /* PCG iterations (max over right-hand sides)    */
+  int converged;           /* 1 if every column reached tolerance           */
+  double rel_residual;     /* true ||b - A x|| / ||b|| (max over columns)   */
+  double setup_ms;         /* hierarchy build time of this call (0 if reused)*/
+  double solve_ms;         /* device time of the iteration                  */
+  int num_levels;
+  double grid_complexity;
+  double operator_complexity;
+} fem_solve_info;
+
+/* FEMSolver::solveFEM(x_h, b_h), src/FEMSolver.cu:58-93, with AMG::solve,
+ * src/core/cuda/amg.cu:173-200 and CG_Flex_Cycle, src/core/cuda/cgcycle.cu:7-69.
+ * b, x: nv x nrhs, row-major (vertex-major), user numbering; x holds the
+ * initial guess on entry and the solution on return.  on_device != 0: b and x
+ * are device pointers.  solverType 0 applies one V-cycle; 1 runs PCG to
+ * `tolerance` on the true FP64 residual.  Returns FEM_ERR_NOT_CONVERGED (with
+ * x filled) if maxIters ends the iteration first. */
+int fem_solve(fem_handle* h, const double* b, double* x, int nrhs, int on_device,
+              fem_solve_info* info);
+
+/* ---- adjoint (no counterpart in the reference) -----------------------------*/
+/* dLdkappa[e] = -|T_e| grad(lambda)_e . grad(u)_e with K lambda = dLdu, solved
+ * on the hierarchy of the last setup (K symmetric).  lambda_out may be NULL.
+ * u, dLdu, lambda_out: nv, user numbering; dLdkappa: ne, user element order. */
+int fem_adjoint_grad(fem_handle* h, const double* u, const double* dLdu,
+                     double* dLdkappa, double* lambda_out, int on_device,
+                     fem_solve_info* info);
+
+/* Element gradient only: dLdkappa[e] = -|T_e| grad(a)_e . grad(u)_e. */
+int fem_element_grad(fem_handle* h, const double* a, const double* u,
+                     double* dLdkappa, int on_device);
+
+/* y = A x with the assembled FP64 operator (user numbering, nv x nrhs). */
+int fem_apply_operator(fem_handle* h, const double* x, double* y, int nrhs, int on_device);
+
+/* z = M r: one application of the V-cycle preconditioner (user numbering). */
+int fem_apply_preconditioner(fem_handle* h, const double* r, double* z, int on_device);
+
+/* ---- hierarchy inspection (tests and diagnostics) ---------------------------*/
+int fem_get_num_levels(const fem_handle* h);
+/* Level `lvl` operator in the library's internal numbering: rows, nnz. */
+int fem_get_level_info(const fem_handle* h, int lvl, int64_t* n, int64_t* nnz);
+/* Copy level data (internal numbering) to host arrays; NULL skips an item.
+ * aggregates: n entries (coarse id of each row; absent on the coarsest). */
+int fem_get_level_matrix(const fem_handle* h, int lvl, int32_t* row_ptr, int32_t* cols,
+                         double* vals, int32_t* aggregates);
+/* Prolongator of level lvl -> lvl+1 (n_lvl x n_lvl+1 CSR); NULL skips. */
+int fem_get_level_prolongator(const fem_handle* h, int lvl, int64_t* nnz,
+                              int32_t* row_ptr, int32_t* cols, double* vals);
+/* internal -> user vertex map (nv entries). */
+int fem_get_permutation(const fem_handle* h, int32_t* perm);
+
+/* Run the device randomized distance-k MIS + aggregation on a host CSR graph
+ * (no diagonal) with the given parameters; writes the aggregate id of every
+ * node and returns the aggregate count in *n_agg.  Exposes the kernels of
+ * RandMIS_Aggregator / misHelpers::aggregateGraph
+ * (src/core/cuda/misHelpers.cu:513-601, randomizedMIS_GPU.cu:154-272). */
+int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj,
+                         int depth, int min_size, int rule, unsigned long long seed,
+                         int32_t* aggregate, int32_t* n_agg);
+
+/* ---- profiling -------------------------------------------------------------*/
+/* Time each launch of the named kernel family with HIP events on the handle's
+ * stream while enabled.  Families: "jacobi_l0", "spmv_l0", "pcg_spmv". */
+int fem_profile_enable(fem_handle* h, int enable);
+int fem_profile_get(const fem_handle* h, const char* family, int64_t* launches,
+                    double* total_ms, double* bytes_per_launch);
+
+/* ---- host IO (no GPU needed) -----------------------------------------------*/
+/* TetMesh::read (src/core/cuda/tetmesh.cu:251-376): float-parsed coordinates,
+ * 0/1-based detection, optional label column.  Arrays are malloc'd; free
+ * with fem_free.  labels is filled with zeros if the file has none. */
+int fem_read_tetgen(const char* base, int64_t* nv, double** xyz, int64_t* ne,
+                    int32_t** tets, int32_t** labels);
+/* TriMesh::read subset: ASCII PLY and OFF vertices + triangle faces
+ * (src/core/aggmis/cuda/TriMesh_io.cu:259-502,692-708). */
+int fem_read_trimesh(const char* filename, int64_t* nv, double** xyz, int64_t* ne,
+                     int32_t** tris);
+void fem_free(void* p);
+
+/* FEMSolver::readMatlabArray, src/FEMSolver.cu:358-449.  With out == NULL
+ * only *n is set.  Returns 0 or -1 (reference codes). */
+int fem_read_matlab_array(const char* filename, double* out, int64_t* n);
+/* FEMSolver::writeMatlabArray, src/FEMSolver.cu:451-517 (byte-identical). */
+int fem_write_matlab_array(const char* filename, const double* x, int64_t n);
+/* MAT v5 sparse reader half of readMatlabSparseMatrix (src/FEMSolver.cu:177-311):
+ * COO entries rounded to float as the reference does when round_to_float != 0.
+ * With rows == NULL only *nnz, *nrows, *ncols are set. */
+int fem_read_matlab_sparse_coo(const char* filename, int round_to_float, int64_t* nrows,
+                               int64_t* ncols, int64_t* nnz, int32_t* rows, int32_t* cols,
+                               double* vals);
+/* FEMSolver::writeVTK, src/FEMSolver.cu:519-582 (tets when dim==3). */
+int fem_write_vtk(const char* fname, int dim, int64_t nv, const double* xyz, int64_t ne,
+                  const int32_t* conn, const double* values, int64_t nvalues);
+
+/* Library version string. */
+const char* fem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEM_MI355X_H */

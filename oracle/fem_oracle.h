@@ -1,0 +1,22 @@
+/* fem_oracle.h -- CPU restatement of the reference algorithm (test infrastructure only). */
+#ifndef FEM_ORACLE_H
+#define FEM_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+int64_t orc_pattern(int npe, int64_t nv, int64_t ne, const int32_t* conn, int64_t* rp, int32_t* ci);
+void orc_gauss_lobatto_jacobi(int q, double a, double b, double* z, double* w);
+void orc_gauss_radau_jacobi(int q, double a, double b, double* z, double* w);
+void orc_tet_mass_integrals(double out[10]);
+void orc_tri_mass_integrals(double out[6]);
+void orc_tet_element(const double* X, double co, const double integ[10], double K[16], double M[16]);
+void orc_tri_element(const double* X, int signed_mass, const double integ[6], double K[9], double M[9]);
+int orc_assemble(int dim, int64_t nv, const double* xyz, int64_t ne, const int32_t* conn, const double* kappa,
+                 double mass_coeff, int signed_mass, const int64_t* rp, const int32_t* ci, double* vals);
+void orc_element_grad(int dim, const double* xyz, int64_t ne, const int32_t* conn, const double* a, const double* u,
+                      double* out);
+#ifdef __cplusplus
+}
+#endif
+#endif

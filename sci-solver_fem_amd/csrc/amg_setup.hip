@@ -1,0 +1,685 @@
+// Smoothed-aggregation AMG setup on the device.
+//
+// Reference algorithm (src/core/cuda/amg.cu:79-144, SmoothedMG_AMG_Level::
+// createNextLevel src/core/cuda/smoothedMG_amg_level.cu:403-494):
+//   per level: randomized distance-2 MIS (randomizedMIS_GPU.cu:154-272) ->
+//   node-to-aggregate allocation with removal of aggregates smaller than 9
+//   (misHelpers.cu:13-75,513-601,744) -> tentative T(i, agg(i)) = 1 ->
+//   P = (I - omega D^-1 A) T (smoothedMG_amg_level.cu:321-387) -> R = P^T ->
+//   A_c = R (A P) (:390-400); the next level aggregates the induced aggregate
+//   graph (misHelpers.cu:1078); the coarsest level (< topSize rows) is solved
+//   directly (amg.cu:101-107).
+// The MIS kernels below keep the reference's arithmetic (Tausworthe steps,
+// value % 1000000, 1000001 for members, ties to the larger origin) with the
+// time() seed replaced by a SplitMix64 stream, so the aggregation is a
+// deterministic function of (graph, seed) and matches the oracle bit for bit.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "fem_internal.hpp"
+#include "kernels.hpp"
+
+namespace fem {
+
+namespace {
+
+constexpr int kMisThreads = 32768;  // Generate_Randoms_Kernel: 128 x 256 generator threads
+constexpr int kMisInMis = 1000001;
+
+inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// seed of generator thread i (replaces rand() after srand(time(NULL)))
+inline uint32_t mis_seed(uint64_t seed, uint32_t i) {
+  return (uint32_t)(mix64(seed * 0x9E3779B97F4A7C15ull + (uint64_t)(i + 1) * 0xD1B54A32D192ED03ull) >> 32);
+}
+
+__device__ __forceinline__ uint32_t taus_step(uint32_t z) {
+  uint32_t b = (((z << 13) ^ z) >> 19);
+  return ((z & 0xffffffffu) << 12) ^ b;
+}
+
+__global__ void mis_randoms_kernel(int64_t size, int iterations, uint32_t* __restrict__ randoms,
+                                   const uint32_t* __restrict__ seeds) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kMisThreads) return;
+  uint32_t z = seeds[idx];
+  int64_t offset = idx;
+  for (int i = 0; i < iterations; ++i) {
+    if (offset < size) {
+      z = taus_step(z);
+      randoms[offset] = z;
+      offset += kMisThreads;
+    }
+  }
+}
+
+__global__ void mis_first_init_kernel(int64_t size, const uint32_t* __restrict__ randoms, int* __restrict__ best,
+                                      int* __restrict__ origin) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= size) return;
+  origin[idx] = (int)idx;
+  best[idx] = (int)(randoms[idx] % 1000000u);
+}
+
+__global__ void mis_init_kernel(int64_t size, uint32_t* __restrict__ randoms, int* __restrict__ best,
+                                int* __restrict__ origin, const int* __restrict__ mis) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= size) return;
+  uint32_t z = taus_step(randoms[idx]);
+  origin[idx] = (int)idx;
+  int status = mis[idx];
+  best[idx] = (status == -1) ? (int)(z % 1000000u) : (status == 1 ? kMisInMis : 0);
+  randoms[idx] = z;
+}
+
+// one hop of (best value, origin) max propagation; final hop also decides
+template <bool kFinal>
+__global__ void mis_iterate_kernel(int64_t size, const int* __restrict__ origin_in, int* __restrict__ origin_out,
+                                   const int* __restrict__ best_in, int* __restrict__ best_out,
+                                   const int32_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj,
+                                   int* __restrict__ mis, int* __restrict__ incomplete) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= size) return;
+  int best = best_in[idx];
+  int origin = origin_in[idx];
+  if (best < kMisInMis) {
+    for (int i = adj_ptr[idx]; i < adj_ptr[idx + 1]; ++i) {
+      int nb = adj[i];
+      int ch = best_in[nb];
+      int cho = origin_in[nb];
+      if (ch > 0 && ch == best && cho > origin) origin = cho;
+      if (ch > best) {
+        best = ch;
+        origin = cho;
+      }
+    }
+  }
+  if (!kFinal) {
+    best_out[idx] = best;
+    origin_out[idx] = origin;
+  } else {
+    int st = -1;
+    if (origin == (int)idx) st = 1;
+    else if (best == kMisInMis) st = 0;
+    mis[idx] = st;
+    if (st == -1) *incomplete = 1;
+  }
+}
+
+// partIn[i] = aggregate id of a root (rank of the root in index order), -1 else
+__global__ void roots_kernel(int64_t n, const int* __restrict__ mis, const int32_t* __restrict__ scan,
+                             int* __restrict__ part, int* __restrict__ aggregated) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool root = mis[i] == 1;
+  part[i] = root ? scan[i] : -1;  // exclusive scan == inclusive - 1 for roots
+  aggregated[i] = root ? 1 : 0;
+}
+
+// rule 0: aggregate of the first aggregated neighbour in adjacency order
+//         (what the reference's allocateNodesKernel computes);
+// rule 1: most frequent neighbouring aggregate, ties to the smallest id.
+__global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__ adj_ptr,
+                                const int32_t* __restrict__ adj, const int* __restrict__ part_in,
+                                int* __restrict__ part_out, int* __restrict__ aggregated) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (aggregated[i] != 0) {
+    part_out[i] = part_in[i];
+    return;
+  }
+  int add_to = -1;
+  if (rule == 0) {
+    for (int p = adj_ptr[i]; p < adj_ptr[i + 1]; ++p) {
+      int c = part_in[adj[p]];
+      if (c != -1) {
+        add_to = c;
+        break;
+      }
+    }
+  } else {
+    int best_cnt = 0;
+    for (int p = adj_ptr[i]; p < adj_ptr[i + 1]; ++p) {
+      int c = part_in[adj[p]];
+      if (c == -1) continue;
+      bool seen = false;
+      for (int q = adj_ptr[i]; q < p; ++q)
+        if (part_in[adj[q]] == c) { seen = true; break; }
+      if (seen) continue;
+      int cnt = 0;
+      for (int q = p; q < adj_ptr[i + 1]; ++q) cnt += (part_in[adj[q]] == c);
+      if (cnt > best_cnt || (cnt == best_cnt && c < add_to)) {
+        best_cnt = cnt;
+        add_to = c;
+      }
+    }
+  }
+  part_out[i] = add_to;
+  if (add_to != -1) aggregated[i] = 1;
+}
+
+__global__ void count_unallocated_kernel(int64_t n, const int* __restrict__ aggregated, unsigned* __restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && aggregated[i] == 0) atomicAdd(cnt, 1u);
+}
+
+__global__ void part_sizes_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ sizes) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && part[i] >= 0) atomicAdd(&sizes[part[i]], 1);
+}
+
+__global__ void runt_stencil_kernel(int64_t np, const int* __restrict__ sizes, int min_size, int32_t* __restrict__ st) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) st[p] = sizes[p] < min_size ? 1 : 0;
+}
+
+// removeRuntyPartsKernel: drop runts, renumber survivors densely
+__global__ void runt_apply_kernel(int64_t n, int* __restrict__ part, const int32_t* __restrict__ st,
+                                  const int32_t* __restrict__ excl, int* __restrict__ aggregated) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int p = part[i];
+  if (st[p]) {
+    part[i] = -1;
+    aggregated[i] = 0;
+  } else {
+    part[i] = p - excl[p];  // excl = exclusive count of removed ids below p
+    aggregated[i] = 1;
+  }
+}
+
+__global__ void stuck_flag_kernel(int64_t n, const int* __restrict__ aggregated, int32_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = aggregated[i] == 0 ? 1 : 0;
+}
+
+__global__ void stuck_assign_kernel(int64_t n, int base, const int32_t* __restrict__ flag,
+                                    const int32_t* __restrict__ excl, int* __restrict__ part) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) part[i] = base + excl[i];
+}
+
+__global__ void max_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ mx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(mx, part[i]);
+}
+
+unsigned read_counter(DevBuf<unsigned>& c, hipStream_t s) {
+  unsigned v = 0;
+  HIP_CHECK(hipMemcpyAsync(&v, c.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return v;
+}
+
+void randomized_mis(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth, unsigned long long seed,
+                    int* mis, hipStream_t s) {
+  std::vector<uint32_t> seeds_h(kMisThreads);
+  for (int i = 0; i < kMisThreads; ++i) seeds_h[i] = mis_seed(seed, (uint32_t)i);
+  DevBuf<uint32_t> seeds;
+  seeds.upload(seeds_h.data(), kMisThreads, s);
+  DevBuf<uint32_t> randoms(n);
+  DevBuf<int> best_a(n), best_b(n), orig_a(n), orig_b(n);
+  DevBuf<int> incomplete(1);
+  dev_fill_i32(mis, -1, n, s);
+  int iterations = (int)((n + kMisThreads - 1) / kMisThreads);
+  mis_randoms_kernel<<<kMisThreads / 256, 256, 0, s>>>(n, iterations, randoms.get(), seeds.get());
+  HIP_LAUNCH_CHECK();
+  mis_first_init_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, randoms.get(), best_a.get(), orig_a.get());
+  HIP_LAUNCH_CHECK();
+  bool first = true;
+  for (int round = 0;; ++round) {
+    if (round > 100000) fail(FEM_ERR_NUMERIC, "randomized MIS did not complete");
+    if (!first) {
+      mis_init_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, randoms.get(), best_a.get(), orig_a.get(), mis);
+      HIP_LAUNCH_CHECK();
+    }
+    first = false;
+    HIP_CHECK(hipMemsetAsync(incomplete.get(), 0, sizeof(int), s));
+    int* bi = best_a.get(); int* bo = best_b.get();
+    int* oi = orig_a.get(); int* oo = orig_b.get();
+    for (int i = 0; i < depth; ++i) {
+      if (i < depth - 1) {
+        mis_iterate_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(n, oi, oo, bi, bo, adj_ptr, adj, mis,
+                                                                  incomplete.get());
+        std::swap(bi, bo);
+        std::swap(oi, oo);
+      } else {
+        mis_iterate_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(n, oi, oo, bi, bo, adj_ptr, adj, mis,
+                                                                 incomplete.get());
+      }
+      HIP_LAUNCH_CHECK();
+    }
+    // Initialize_Kernel rewrites best/origin completely each round, so which
+    // buffer pair is "in" at the start of a round does not change the result.
+    int inc = 0;
+    HIP_CHECK(hipMemcpyAsync(&inc, incomplete.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (!inc) break;
+  }
+}
+
+// ---- prolongator, diagonal, power iteration ---------------------------------------
+
+__global__ void diag_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const double* __restrict__ v, double* __restrict__ dinv, int* __restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  int lo = rp[i], hi = rp[i + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ci[mid] < (int)i) lo = mid + 1; else hi = mid;
+  }
+  if (lo < rp[i + 1] && ci[lo] == (int)i) d = v[lo];
+  if (!(d > 0.0)) {
+    atomicAdd(bad, 1);
+    dinv[i] = 0.0;
+  } else {
+    dinv[i] = 1.0 / d;
+  }
+}
+
+// P(i, J) = delta(J, agg(i)) - omega / d_i * sum_{k: agg(k) = J} a_ik, summed in
+// ascending k then the tentative 1 (order of the reference's reduce_by_key).
+__global__ void prolongator_kernel(int64_t n, const int32_t* __restrict__ a_rp, const int32_t* __restrict__ a_ci,
+                                   const double* __restrict__ a_v, const double* __restrict__ dinv,
+                                   const int32_t* __restrict__ agg, double omega, const int32_t* __restrict__ p_rp,
+                                   const int32_t* __restrict__ p_ci, double* __restrict__ p_v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = p_rp[i], len = p_rp[i + 1] - o;
+  for (int t = 0; t < len; ++t) p_v[o + t] = 0.0;
+  const double d = (dinv[i] != 0.0) ? 1.0 / dinv[i] : 0.0;
+  auto slot = [&](int J) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (p_ci[o + mid] < J) lo = mid + 1; else hi = mid;
+    }
+    return o + lo;
+  };
+  for (int q = a_rp[i]; q < a_rp[i + 1]; ++q) p_v[slot(agg[a_ci[q]])] += (-omega * a_v[q]) / d;
+  p_v[slot(agg[i])] += 1.0;
+}
+
+__global__ void dinv_scale_kernel(int64_t n, const double* __restrict__ dinv, double* __restrict__ w) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] *= dinv[i];
+}
+
+__global__ void sumsq_kernel(int64_t n, const double* __restrict__ v, double* __restrict__ part) {
+  __shared__ double s_buf[kNT];
+  double a = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) a += v[i] * v[i];
+  double t = block_sum(a, s_buf);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void scale_kernel(int64_t n, double a, double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= a;
+}
+
+__global__ void hash_vector_kernel(int64_t n, unsigned long long seed, double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = (uint64_t)seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  v[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+double norm2(const double* v, int64_t n, hipStream_t s) {
+  const int nb = 256;
+  DevBuf<double> part(nb);
+  sumsq_kernel<<<nb, kNT, 0, s>>>(n, v, part.get());
+  HIP_LAUNCH_CHECK();
+  std::vector<double> h = part.to_host(s);
+  double t = 0.0;
+  for (double x : h) t += x;
+  return std::sqrt(t);
+}
+
+// Spectral radius of D^-1 A by power iteration (upper part of the spectrum).
+double estimate_rho(const Csr& A, const double* dinv, hipStream_t s, int iters = 15) {
+  const int64_t n = A.n;
+  DevBuf<double> v(n), w(n);
+  hash_vector_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 12345ull, v.get());
+  HIP_LAUNCH_CHECK();
+  double rho = 1.0;
+  double nv = norm2(v.get(), n, s);
+  scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 1.0 / nv, v.get());
+  for (int it = 0; it < iters; ++it) {
+    spmv64(A, v.get(), w.get(), s);
+    dinv_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, dinv, w.get());
+    HIP_LAUNCH_CHECK();
+    double nw = norm2(w.get(), n, s);
+    rho = nw;
+    if (!(nw > 0)) break;
+    scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 1.0 / nw, w.get());
+    std::swap(v, w);
+  }
+  return rho;
+}
+
+__global__ void lower_bound_u32_kernel(int64_t m, const uint32_t* __restrict__ keys, int64_t nk,
+                                       int32_t* __restrict__ ptr) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > m) return;
+  int64_t lo = 0, hi = nk;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)keys[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  ptr[v] = (int32_t)lo;
+}
+
+void lower_bound_u32(int64_t m, const uint32_t* keys, int64_t nk, int32_t* ptr, hipStream_t s) {
+  lower_bound_u32_kernel<<<grid_for(m + 1, 256), 256, 0, s>>>(m, keys, nk, ptr);
+  HIP_LAUNCH_CHECK();
+}
+
+// members of each aggregate, ascending node id: CSR (nagg rows)
+void aggregate_members(const int32_t* agg, int64_t n, int64_t nagg, Csr& M, hipStream_t s) {
+  DevBuf<uint32_t> key(n), key2(n);
+  DevBuf<int32_t> idx(n);
+  HIP_CHECK(hipMemcpyAsync(key.get(), agg, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  dev_iota_i32(idx.get(), n, s);
+  M.n = nagg;
+  M.m = n;
+  M.nnz = n;
+  M.ci.resize(n);
+  dev_sort_pairs_u32_i32(key.get(), key2.get(), idx.get(), M.ci.get(), n, 32, s);
+  M.rp.resize(nagg + 1);
+  lower_bound_u32(nagg, key2.get(), n, M.rp.get(), s);
+}
+
+// Dense inverse of a small SPD matrix (Cholesky; LU with partial pivoting if
+// Cholesky breaks down).  a is row-major n x n, replaced by its inverse.
+bool dense_inverse(std::vector<double>& a, int64_t n) {
+  std::vector<double> L(a);
+  bool spd = true;
+  for (int64_t j = 0; j < n && spd; ++j) {
+    double d = L[j * n + j];
+    for (int64_t k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
+    if (!(d > 0.0)) { spd = false; break; }
+    d = std::sqrt(d);
+    L[j * n + j] = d;
+    for (int64_t i = j + 1; i < n; ++i) {
+      double v = L[i * n + j];
+      for (int64_t k = 0; k < j; ++k) v -= L[i * n + k] * L[j * n + k];
+      L[i * n + j] = v / d;
+    }
+  }
+  std::vector<double> inv(n * n, 0.0);
+  if (spd) {
+    // solve L L^T X = I column by column
+    std::vector<double> y(n);
+    for (int64_t c = 0; c < n; ++c) {
+      for (int64_t i = 0; i < n; ++i) {
+        double v = (i == c) ? 1.0 : 0.0;
+        for (int64_t k = 0; k < i; ++k) v -= L[i * n + k] * y[k];
+        y[i] = v / L[i * n + i];
+      }
+      for (int64_t i = n - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int64_t k = i + 1; k < n; ++k) v -= L[k * n + i] * inv[k * n + c];
+        inv[i * n + c] = v / L[i * n + i];
+      }
+    }
+    a.swap(inv);
+    return true;
+  }
+  // LU with partial pivoting
+  std::vector<double> U(a);
+  std::vector<int64_t> piv(n);
+  for (int64_t i = 0; i < n; ++i) piv[i] = i;
+  for (int64_t j = 0; j < n; ++j) {
+    int64_t p = j;
+    for (int64_t i = j + 1; i < n; ++i)
+      if (std::fabs(U[i * n + j]) > std::fabs(U[p * n + j])) p = i;
+    if (U[p * n + j] == 0.0) return false;
+    if (p != j) {
+      for (int64_t k = 0; k < n; ++k) std::swap(U[p * n + k], U[j * n + k]);
+      std::swap(piv[p], piv[j]);
+    }
+    for (int64_t i = j + 1; i < n; ++i) {
+      double f = U[i * n + j] / U[j * n + j];
+      U[i * n + j] = f;
+      for (int64_t k = j + 1; k < n; ++k) U[i * n + k] -= f * U[j * n + k];
+    }
+  }
+  std::vector<double> y(n);
+  for (int64_t c = 0; c < n; ++c) {
+    for (int64_t i = 0; i < n; ++i) {
+      double v = (piv[i] == c) ? 1.0 : 0.0;
+      for (int64_t k = 0; k < i; ++k) v -= U[i * n + k] * y[k];
+      y[i] = v;
+    }
+    for (int64_t i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int64_t k = i + 1; k < n; ++k) v -= U[i * n + k] * inv[k * n + c];
+      inv[i * n + c] = v / U[i * n + i];
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
+void to_f32(Csr& A, hipStream_t s) {
+  A.v32.resize(A.nnz > 0 ? A.nnz : 1);
+  dev_f64_to_f32(A.v64.get(), A.v32.get(), A.nnz, s);
+}
+
+}  // namespace
+
+void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth, int min_size, int rule,
+                     unsigned long long seed, int32_t* agg_out, int64_t* n_agg, hipStream_t s) {
+  if (n == 0) {
+    *n_agg = 0;
+    return;
+  }
+  DevBuf<int> mis(n);
+  randomized_mis(n, adj_ptr, adj, depth < 1 ? 1 : depth, seed, mis.get(), s);
+  // roots numbered by rank (aggregateGraph: inclusive scan + ifLabelOne)
+  DevBuf<int32_t> scan(n + 1);
+  DevBuf<int> part_in(n), part_out(n), aggregated(n);
+  // mis is 0/1 after completion, so its exclusive scan numbers the roots
+  dev_exclusive_scan_i32(mis.get(), scan.get(), n, s);
+  roots_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, mis.get(), scan.get(), part_in.get(), aggregated.get());
+  HIP_LAUNCH_CHECK();
+  DevBuf<unsigned> cnt(1);
+  DevBuf<int> sizes;
+  unsigned prev_unalloc = 0xffffffffu;
+  for (int loop = 0;; ++loop) {
+    if (loop > 1000000) fail(FEM_ERR_NUMERIC, "aggregation did not complete");
+    allocate_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rule, adj_ptr, adj, part_in.get(), part_out.get(),
+                                                     aggregated.get());
+    HIP_LAUNCH_CHECK();
+    std::swap(part_in, part_out);
+    HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned), s));
+    count_unallocated_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, aggregated.get(), cnt.get());
+    HIP_LAUNCH_CHECK();
+    unsigned unalloc = read_counter(cnt, s);
+    if (unalloc == 0) {
+      // removeRuntyParts
+      DevBuf<int> mx(1);
+      dev_fill_i32(mx.get(), -1, 1, s);
+      max_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), mx.get());
+      HIP_LAUNCH_CHECK();
+      int hmx = -1;
+      HIP_CHECK(hipMemcpyAsync(&hmx, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      int64_t np = hmx + 1;
+      sizes.resize(np);
+      dev_fill_i32(sizes.get(), 0, np, s);
+      part_sizes_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), sizes.get());
+      HIP_LAUNCH_CHECK();
+      DevBuf<int32_t> st(np), excl(np + 1);
+      runt_stencil_kernel<<<grid_for(np, 256), 256, 0, s>>>(np, sizes.get(), min_size, st.get());
+      HIP_LAUNCH_CHECK();
+      int64_t removed = dev_exclusive_scan_i32(st.get(), excl.get(), np, s);
+      if (removed == 0) {
+        *n_agg = np;
+        break;
+      }
+      runt_apply_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), st.get(), excl.get(), aggregated.get());
+      HIP_LAUNCH_CHECK();
+      prev_unalloc = 0xffffffffu;
+      continue;
+    }
+    if (unalloc == prev_unalloc) {
+      // Nodes with no aggregated node in reach (a component left without a
+      // root after runt removal): each becomes its own aggregate and the
+      // aggregation ends (the reference loops forever here).
+      DevBuf<int> mx(1);
+      dev_fill_i32(mx.get(), -1, 1, s);
+      max_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), mx.get());
+      HIP_LAUNCH_CHECK();
+      int hmx = -1;
+      HIP_CHECK(hipMemcpyAsync(&hmx, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      DevBuf<int32_t> fl(n), ex(n + 1);
+      stuck_flag_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, aggregated.get(), fl.get());
+      HIP_LAUNCH_CHECK();
+      int64_t nst = dev_exclusive_scan_i32(fl.get(), ex.get(), n, s);
+      stuck_assign_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, hmx + 1, fl.get(), ex.get(), part_in.get());
+      HIP_LAUNCH_CHECK();
+      *n_agg = hmx + 1 + nst;
+      break;
+    }
+    prev_unalloc = unalloc;
+  }
+  HIP_CHECK(hipMemcpyAsync(agg_out, part_in.get(), sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void build_hierarchy(Handle& h) {
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = h.stream;
+  Hierarchy& H = h.H;
+  H.levels.clear();
+  H.cinv64.release();
+  H.cinv32.release();
+  H.precision = h.prm.precision;
+  H.built = false;
+  const int fine_depth = h.prm.randMisParameters % 100;
+  const int min_size = (h.prm.randMisParameters / 10000) % 10;
+  const double omega = h.prm.proOmega;
+
+  // level-0 graph: pattern of A without the diagonal
+  Csr G;
+  spgemm_symbolic(h.A, nullptr, nullptr, 0, nullptr, h.A.n, true, G, s);
+
+  int64_t total_rows = 0, total_nnz = 0;
+  std::unique_ptr<Level> cur(new Level());
+  for (int l = 0;; ++l) {
+    Level& L = *cur;
+    const Csr& A = (l == 0) ? h.A : L.A;
+    const int64_t n = A.n;
+    total_rows += n;
+    total_nnz += A.nnz;
+    L.dinv64.resize(n);
+    {
+      DevBuf<int> bad(1);
+      dev_fill_i32(bad.get(), 0, 1, s);
+      diag_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), L.dinv64.get(), bad.get());
+      HIP_LAUNCH_CHECK();
+      int hb = 0;
+      HIP_CHECK(hipMemcpyAsync(&hb, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (hb) fail(FEM_ERR_NUMERIC, "level " + std::to_string(l) + " has " + std::to_string(hb) +
+                                        " rows without a positive diagonal");
+    }
+    const bool last = (n < h.prm.topSize) || (l + 1 >= h.prm.maxLevels) || n <= 1;
+    int64_t nagg = 0;
+    if (!last) {
+      L.agg.resize(n);
+      unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
+      aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
+                      &nagg, s);
+    }
+    if (last || nagg >= n || nagg == 0) {
+      // coarsest level: dense FP64 inverse
+      if (n > 8192)
+        fail(FEM_ERR_CAPACITY, "coarsest level has " + std::to_string(n) +
+                                   " rows (limit 8192); raise maxLevels or check the aggregation");
+      L.agg.release();
+      std::vector<int32_t> rp(n + 1), ci(A.nnz);
+      std::vector<double> v(A.nnz);
+      A.rp.download(rp.data(), n + 1, s);
+      A.ci.download(ci.data(), A.nnz, s);
+      A.v64.download(v.data(), A.nnz, s);
+      HIP_CHECK(hipStreamSynchronize(s));
+      std::vector<double> dense(n * n, 0.0);
+      for (int64_t i = 0; i < n; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q) dense[i * n + ci[q]] += v[q];
+      if (!dense_inverse(dense, n)) fail(FEM_ERR_NUMERIC, "coarsest-level matrix is singular");
+      H.cinv64.upload(dense.data(), n * n, s);
+      if (H.precision == 1) {
+        H.cinv32.resize(n * n);
+        dev_f64_to_f32(H.cinv64.get(), H.cinv32.get(), n * n, s);
+      }
+      H.coarse = l;
+      H.nc = n;
+      H.levels.push_back(std::move(cur));
+      break;
+    }
+    L.nc = nagg;
+    L.rho = estimate_rho(A, L.dinv64.get(), s);
+    // prolongator pattern: columns agg(k) for k in row i of A
+    spgemm_symbolic(A, nullptr, nullptr, 0, L.agg.get(), nagg, false, L.P, s);
+    L.P.v64.resize(L.P.nnz);
+    prolongator_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), L.dinv64.get(),
+                                                        L.agg.get(), omega, L.P.rp.get(), L.P.ci.get(),
+                                                        L.P.v64.get());
+    HIP_LAUNCH_CHECK();
+    csr_transpose(L.P, L.R, s);
+    // A_c = R (A P)
+    std::unique_ptr<Level> next(new Level());
+    {
+      Csr AP;
+      spgemm_symbolic(A, L.P.rp.get(), L.P.ci.get(), L.P.n, nullptr, nagg, false, AP, s);
+      spgemm_numeric(A, L.P, AP, s);
+      spgemm_symbolic(L.R, AP.rp.get(), AP.ci.get(), AP.n, nullptr, nagg, false, next->A, s);
+      spgemm_numeric(L.R, AP, next->A, s);
+    }
+    // next graph: induced aggregate graph (aggregates sharing an edge)
+    {
+      Csr M;
+      aggregate_members(L.agg.get(), n, nagg, M, s);
+      Csr G2;
+      spgemm_symbolic(M, G.rp.get(), G.ci.get(), G.n, L.agg.get(), nagg, true, G2, s);
+      G = std::move(G2);
+    }
+    H.levels.push_back(std::move(cur));
+    cur = std::move(next);
+  }
+  H.grid_complexity = (double)total_rows / (double)h.A.n;
+  H.operator_complexity = (double)total_nnz / (double)h.A.nnz;
+  if (H.precision == 1) {
+    for (size_t l = 0; l < H.levels.size(); ++l) {
+      Level& L = *H.levels[l];
+      Csr& A = (l == 0) ? h.A : L.A;
+      to_f32(A, s);
+      L.dinv32.resize(L.dinv64.size());
+      dev_f64_to_f32(L.dinv64.get(), L.dinv32.get(), L.dinv64.size(), s);
+      if ((int)l != H.coarse) {
+        to_f32(L.P, s);
+        to_f32(L.R, s);
+      }
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  H.built = true;
+  H.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace fem

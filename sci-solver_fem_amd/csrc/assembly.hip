@@ -1,0 +1,526 @@
+// Mesh reordering, P1 sparsity pattern and deterministic P1 assembly.
+//
+// Reference behaviour being replaced:
+//   pattern  : TetMesh::need_neighbors + tetmesh2ell / trimesh2ell
+//              (src/core/cuda/tetmesh.cu:112-170, src/core/cuda/cutil.cu:169-230,298-348):
+//              row i holds i and every vertex sharing an element with i.
+//   values   : element_loop_3d_kernel / element_loop_kernel
+//              (src/core/cuda/perform_element_loop_3D.cuh:481-627,
+//               src/core/cuda/perform_element_loop_2D.cuh:288-372):
+//              K_e = co * |T| grad(phi_a).grad(phi_b) + M_e, scattered with FP64 CAS atomics.
+//
+// Here the scatter is replaced by a gather: one thread owns one matrix row,
+// walks the elements incident to its vertex in ascending element order and
+// sums their contributions into the row in LDS.  No float atomics, a fixed
+// summation order (bit-reproducible) and A is bitwise symmetric because both
+// rows of a pair see the same element contributions in the same order.
+#include "fem_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fem {
+
+namespace {
+
+constexpr int kRowBlock = 64;   // threads (rows) per block for row-owner kernels
+constexpr int kRowCap = 64;     // LDS row capacity; longer rows use the global path
+
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+__device__ __forceinline__ uint64_t spread2(uint64_t v) {  // 32 bits -> every 2nd bit
+  v &= 0xffffffffull;
+  v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+  v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+
+__global__ void morton_kernel(int64_t nv, int dim, const double* __restrict__ xyz, double x0, double y0,
+                              double z0, double sx, double sy, double sz, uint64_t* __restrict__ key,
+                              int32_t* __restrict__ idx) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double x = (xyz[3 * v + 0] - x0) * sx;
+  double y = (xyz[3 * v + 1] - y0) * sy;
+  double z = (xyz[3 * v + 2] - z0) * sz;
+  uint64_t k;
+  if (dim == 3) {
+    double m = 2097151.0;  // 2^21 - 1
+    uint64_t qx = (uint64_t)fmin(fmax(x * m, 0.0), m);
+    uint64_t qy = (uint64_t)fmin(fmax(y * m, 0.0), m);
+    uint64_t qz = (uint64_t)fmin(fmax(z * m, 0.0), m);
+    k = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+  } else {
+    double m = 4294967295.0;
+    uint64_t qx = (uint64_t)fmin(fmax(x * m, 0.0), m);
+    uint64_t qy = (uint64_t)fmin(fmax(y * m, 0.0), m);
+    k = spread2(qx) | (spread2(qy) << 1);
+  }
+  key[v] = k;
+  idx[v] = (int32_t)v;
+}
+
+__global__ void invert_perm_kernel(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ iperm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) iperm[perm[i]] = (int32_t)i;
+}
+
+__global__ void coords_kernel(int64_t nv, const double* __restrict__ xyz, const int32_t* __restrict__ perm,
+                              double4* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int64_t u = perm[i];
+  out[i] = make_double4(xyz[3 * u], xyz[3 * u + 1], xyz[3 * u + 2], 0.0);
+}
+
+// relabel element vertices and compute the element sort key (lowest vertex)
+__global__ void relabel_elems_kernel(int64_t ne, int npe, const int32_t* __restrict__ conn_user,
+                                     const int32_t* __restrict__ iperm, int4* __restrict__ conn_tmp,
+                                     uint32_t* __restrict__ key, int32_t* __restrict__ idx) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int v[4] = {-1, -1, -1, -1};
+  int mn = 0x7fffffff;
+  for (int a = 0; a < npe; ++a) {
+    v[a] = iperm[conn_user[npe * e + a]];
+    mn = min(mn, v[a]);
+  }
+  conn_tmp[e] = make_int4(v[0], v[1], v[2], v[3]);
+  key[e] = (uint32_t)mn;
+  idx[e] = (int32_t)e;
+}
+
+__global__ void permute_elems_kernel(int64_t ne, const int32_t* __restrict__ eperm, const int4* __restrict__ conn_tmp,
+                                     const double* __restrict__ kappa_user, int4* __restrict__ conn,
+                                     double* __restrict__ kappa) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int64_t o = eperm[e];
+  conn[e] = conn_tmp[o];
+  kappa[e] = kappa_user ? kappa_user[o] : 1.0;
+}
+
+__global__ void v2e_keys_kernel(int64_t ne, int npe, const int4* __restrict__ conn, uint32_t* __restrict__ key,
+                                int32_t* __restrict__ val) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int4 c = conn[e];
+  int v[4] = {c.x, c.y, c.z, c.w};
+  for (int a = 0; a < npe; ++a) {
+    key[npe * e + a] = (uint32_t)v[a];
+    val[npe * e + a] = (int32_t)(4 * e + a);
+  }
+}
+
+// ptr[v] = first position of key v in the sorted key array (lower bound)
+__global__ void lower_bound_kernel(int64_t nv, const uint32_t* __restrict__ keys, int64_t nk,
+                                   int32_t* __restrict__ ptr) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nv) return;
+  int64_t lo = 0, hi = nk;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)keys[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  ptr[v] = (int32_t)lo;
+}
+
+// Insert c into the ascending list buf[0..len); returns the new length (cap ends with -1).
+__device__ __forceinline__ int sorted_insert(int* buf, int len, int cap, int c) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (buf[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  if (lo < len && buf[lo] == c) return len;
+  if (len >= cap) return -1;
+  for (int t = len; t > lo; --t) buf[t] = buf[t - 1];
+  buf[lo] = c;
+  return len + 1;
+}
+
+// Row pattern: sorted unique vertices of all elements incident to row vertex.
+// Pass 0 counts (cnt[row], or -1 when the row needs more than kRowCap slots),
+// pass 1 writes columns at rp[row].  Rows flagged -1 go through the global path.
+template <bool kFill>
+__global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
+    int64_t nv, int npe, const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
+    const int4* __restrict__ conn, int32_t* __restrict__ cnt, const int32_t* __restrict__ rp,
+    int32_t* __restrict__ ci) {
+  __shared__ int lds[kRowBlock * kRowCap];
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nv) return;
+  if (kFill && cnt[row] < 0) return;
+  int* buf = lds + threadIdx.x * kRowCap;
+  int len = 0;
+  len = sorted_insert(buf, len, kRowCap, (int)row);
+  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1] && len >= 0; ++p) {
+    int4 c = conn[v2e[p] >> 2];
+    int v[4] = {c.x, c.y, c.z, c.w};
+    for (int a = 0; a < npe && len >= 0; ++a) len = sorted_insert(buf, len, kRowCap, v[a]);
+  }
+  if (!kFill) {
+    cnt[row] = len;  // -1 marks overflow
+  } else {
+    int32_t o = rp[row];
+    for (int t = 0; t < len; ++t) ci[o + t] = buf[t];
+  }
+}
+
+// Global path for long rows: upper bound = 1 + npe * incident elements.
+template <bool kFill>
+__global__ void pattern_rows_global_kernel(int64_t nlong, const int32_t* __restrict__ long_rows,
+                                           const int64_t* __restrict__ scratch_off, int npe,
+                                           const int32_t* __restrict__ v2e_ptr,
+                                           const int32_t* __restrict__ v2e, const int4* __restrict__ conn,
+                                           int* __restrict__ scratch, int32_t* __restrict__ cnt,
+                                           const int32_t* __restrict__ rp, int32_t* __restrict__ ci) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlong) return;
+  int row = long_rows[t];
+  int* buf = scratch + scratch_off[t];
+  int cap = (int)(scratch_off[t + 1] - scratch_off[t]);
+  int len = sorted_insert(buf, 0, cap, row);
+  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1]; ++p) {
+    int4 c = conn[v2e[p] >> 2];
+    int v[4] = {c.x, c.y, c.z, c.w};
+    for (int a = 0; a < npe; ++a) len = sorted_insert(buf, len, cap, v[a]);
+  }
+  if (!kFill) {
+    cnt[row] = len;
+  } else {
+    int32_t o = rp[row];
+    for (int q = 0; q < len; ++q) ci[o + q] = buf[q];
+  }
+}
+
+__global__ void collect_long_rows_kernel(int64_t nv, const int32_t* __restrict__ cnt, const int32_t* __restrict__ v2e_ptr,
+                                         int npe, int32_t* __restrict__ long_rows, int64_t* __restrict__ cap,
+                                         unsigned* __restrict__ nlong) {
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nv) return;
+  if (cnt[row] < 0) {
+    unsigned slot = atomicAdd(nlong, 1u);
+    long_rows[slot] = (int32_t)row;
+    cap[slot] = 1 + (int64_t)npe * (v2e_ptr[row + 1] - v2e_ptr[row]);
+  }
+}
+
+// ---- element geometry --------------------------------------------------------
+// P1 gradients of the barycentric basis and the element measure.
+// 3D: e_k = p_k - p_0, det = e1 . (e2 x e3) = 6 vol (signed),
+//     grad phi_1 = (e2 x e3)/det, grad phi_2 = (e3 x e1)/det, grad phi_3 = (e1 x e2)/det,
+//     grad phi_0 = -(sum).
+// 2D: det = e1 x e2 = 2 area (signed).
+struct Geo {
+  double g[4][3];
+  double meas;         // |T|
+  double signed_meas;  // signed |T|
+};
+
+__device__ __forceinline__ void element_geometry(const double4* __restrict__ X, int4 c, int dim, Geo& G) {
+  double4 p0 = X[c.x], p1 = X[c.y], p2 = X[c.z];
+  double e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+  double e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+  if (dim == 3) {
+    double4 p3 = X[c.w];
+    double e3x = p3.x - p0.x, e3y = p3.y - p0.y, e3z = p3.z - p0.z;
+    // cofactor rows
+    double c1x = e2y * e3z - e2z * e3y, c1y = e2z * e3x - e2x * e3z, c1z = e2x * e3y - e2y * e3x;
+    double c2x = e3y * e1z - e3z * e1y, c2y = e3z * e1x - e3x * e1z, c2z = e3x * e1y - e3y * e1x;
+    double c3x = e1y * e2z - e1z * e2y, c3y = e1z * e2x - e1x * e2z, c3z = e1x * e2y - e1y * e2x;
+    double det = e1x * c1x + e1y * c1y + e1z * c1z;
+    double inv = 1.0 / det;
+    G.g[1][0] = c1x * inv; G.g[1][1] = c1y * inv; G.g[1][2] = c1z * inv;
+    G.g[2][0] = c2x * inv; G.g[2][1] = c2y * inv; G.g[2][2] = c2z * inv;
+    G.g[3][0] = c3x * inv; G.g[3][1] = c3y * inv; G.g[3][2] = c3z * inv;
+    for (int d = 0; d < 3; ++d) G.g[0][d] = -(G.g[1][d] + G.g[2][d] + G.g[3][d]);
+    G.signed_meas = det * (1.0 / 6.0);
+    G.meas = fabs(G.signed_meas);
+  } else {
+    double det = e1x * e2y - e1y * e2x;
+    double inv = 1.0 / det;
+    G.g[1][0] = e2y * inv;  G.g[1][1] = -e2x * inv; G.g[1][2] = 0.0;
+    G.g[2][0] = -e1y * inv; G.g[2][1] = e1x * inv;  G.g[2][2] = 0.0;
+    G.g[0][0] = -(G.g[1][0] + G.g[2][0]);
+    G.g[0][1] = -(G.g[1][1] + G.g[2][1]);
+    G.g[0][2] = 0.0;
+    G.g[3][0] = G.g[3][1] = G.g[3][2] = 0.0;
+    G.signed_meas = 0.5 * det;
+    G.meas = fabs(G.signed_meas);
+  }
+}
+
+__device__ __forceinline__ int find_col(const int* cols, int len, int c) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cols[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Row-owner assembly.  acc/cols point to LDS (short rows) or to the row's own
+// slice of the global CSR arrays (long rows).
+__device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool signed_mass, double mass_coeff,
+                                             const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
+                                             const int4* __restrict__ conn, const double4* __restrict__ X,
+                                             const double* __restrict__ kappa, const int* cols, int len,
+                                             double* acc) {
+  const double mden = (dim == 3) ? (1.0 / 20.0) : (1.0 / 12.0);
+  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1]; ++p) {
+    int code = v2e[p];
+    int e = code >> 2, a = code & 3;
+    int4 c = conn[e];
+    Geo G;
+    element_geometry(X, c, dim, G);
+    double kv = kappa[e] * G.meas;
+    double mv = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * mden;
+    int v[4] = {c.x, c.y, c.z, c.w};
+    for (int b = 0; b < npe; ++b) {
+      double s = G.g[a][0] * G.g[b][0] + G.g[a][1] * G.g[b][1] + G.g[a][2] * G.g[b][2];
+      double val = kv * s + mv * (a == b ? 2.0 : 1.0);
+      acc[find_col(cols, len, v[b])] += val;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kRowBlock) assemble_rows_kernel(
+    int64_t nv, int dim, int npe, int signed_mass, double mass_coeff, const int32_t* __restrict__ v2e_ptr,
+    const int32_t* __restrict__ v2e, const int4* __restrict__ conn, const double4* __restrict__ X,
+    const double* __restrict__ kappa, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    double* __restrict__ val) {
+  __shared__ int s_cols[kRowBlock * kRowCap];
+  __shared__ double s_acc[kRowBlock * kRowCap];
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nv) return;
+  int o = rp[row];
+  int len = rp[row + 1] - o;
+  if (len <= kRowCap) {
+    int* cols = s_cols + threadIdx.x * kRowCap;
+    double* acc = s_acc + threadIdx.x * kRowCap;
+    for (int t = 0; t < len; ++t) { cols[t] = ci[o + t]; acc[t] = 0.0; }
+    assemble_row(row, dim, npe, signed_mass != 0, mass_coeff, v2e_ptr, v2e, conn, X, kappa, cols, len, acc);
+    for (int t = 0; t < len; ++t) val[o + t] = acc[t];
+  } else {
+    double* acc = val + o;
+    for (int t = 0; t < len; ++t) acc[t] = 0.0;
+    assemble_row(row, dim, npe, signed_mass != 0, mass_coeff, v2e_ptr, v2e, conn, X, kappa, ci + o, len, acc);
+  }
+}
+
+// dL/dkappa_e = -|T_e| grad(a)_e . grad(u)_e, written in user element order.
+__global__ void element_gradient_kernel(int64_t ne, int dim, int npe, const int4* __restrict__ conn,
+                                        const double4* __restrict__ X, const int32_t* __restrict__ eperm,
+                                        const double* __restrict__ a, const double* __restrict__ u,
+                                        double* __restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int4 c = conn[e];
+  Geo G;
+  element_geometry(X, c, dim, G);
+  int v[4] = {c.x, c.y, c.z, c.w};
+  double ga[3] = {0, 0, 0}, gu[3] = {0, 0, 0};
+  for (int k = 0; k < npe; ++k) {
+    double av = a[v[k]], uv = u[v[k]];
+    for (int d = 0; d < 3; ++d) {
+      ga[d] += av * G.g[k][d];
+      gu[d] += uv * G.g[k][d];
+    }
+  }
+  out[eperm[e]] = -G.meas * (ga[0] * gu[0] + ga[1] * gu[1] + ga[2] * gu[2]);
+}
+
+__global__ void kappa_gather_kernel(int64_t ne, const int32_t* __restrict__ eperm, const double* __restrict__ ku,
+                                    double* __restrict__ k) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) k[e] = ku[eperm[e]];
+}
+
+__global__ void gather_kernel(int64_t n, int k, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                              double* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  int64_t i = t / k, c = t - i * k;
+  out[t] = in[(int64_t)perm[i] * k + c];
+}
+
+__global__ void scatter_kernel(int64_t n, int k, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                               double* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  int64_t i = t / k, c = t - i * k;
+  out[(int64_t)perm[i] * k + c] = in[t];
+}
+
+}  // namespace
+
+void build_mesh_device(Handle& h) {
+  hipStream_t s = h.stream;
+  const int64_t nv = h.nv, ne = h.ne;
+  const int npe = h.dim + 1;
+  if (nv <= 0 || ne <= 0) fail(FEM_ERR_ARG, "mesh has no vertices or no elements");
+  if (4 * ne >= (int64_t)1 << 31 || nv >= (int64_t)1 << 31)
+    fail(FEM_ERR_CAPACITY, "mesh too large for 32-bit element/vertex indices");
+
+  // bounding box on the host copy
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t v = 0; v < nv; ++v)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], h.h_xyz[3 * v + d]);
+      hi[d] = std::max(hi[d], h.h_xyz[3 * v + d]);
+    }
+  double sc[3];
+  for (int d = 0; d < 3; ++d) sc[d] = (hi[d] > lo[d]) ? 1.0 / (hi[d] - lo[d]) : 0.0;
+
+  DevBuf<double> xyz_u;
+  xyz_u.upload(h.h_xyz.data(), (size_t)(3 * nv), s);
+  DevBuf<int32_t> conn_u;
+  conn_u.upload(h.h_conn.data(), (size_t)(npe * ne), s);
+  DevBuf<double> kappa_u;
+  kappa_u.upload(h.h_kappa.data(), (size_t)ne, s);
+
+  // vertex order
+  {
+    DevBuf<uint64_t> key(nv), key2(nv);
+    DevBuf<int32_t> idx(nv);
+    h.perm.resize(nv);
+    h.iperm.resize(nv);
+    morton_kernel<<<grid_for(nv, 256), 256, 0, s>>>(nv, h.dim, xyz_u.get(), lo[0], lo[1], lo[2], sc[0], sc[1],
+                                                    sc[2], key.get(), idx.get());
+    HIP_LAUNCH_CHECK();
+    dev_sort_pairs_u64_i32(key.get(), key2.get(), idx.get(), h.perm.get(), (size_t)nv, 64, s);
+    invert_perm_kernel<<<grid_for(nv, 256), 256, 0, s>>>(nv, h.perm.get(), h.iperm.get());
+    HIP_LAUNCH_CHECK();
+    h.coords.resize(4 * nv);
+    coords_kernel<<<grid_for(nv, 256), 256, 0, s>>>(nv, xyz_u.get(), h.perm.get(), (double4*)h.coords.get());
+    HIP_LAUNCH_CHECK();
+  }
+  // element order: ascending lowest internal vertex (stable)
+  {
+    DevBuf<int32_t> tmp(4 * ne);
+    DevBuf<uint32_t> key(ne), key2(ne);
+    DevBuf<int32_t> idx(ne);
+    relabel_elems_kernel<<<grid_for(ne, 256), 256, 0, s>>>(ne, npe, conn_u.get(), h.iperm.get(),
+                                                           (int4*)tmp.get(), key.get(), idx.get());
+    HIP_LAUNCH_CHECK();
+    h.eperm.resize(ne);
+    dev_sort_pairs_u32_i32(key.get(), key2.get(), idx.get(), h.eperm.get(), (size_t)ne, 32, s);
+    h.conn.resize(4 * ne);
+    h.kappa.resize(ne);
+    permute_elems_kernel<<<grid_for(ne, 256), 256, 0, s>>>(ne, h.eperm.get(), (const int4*)tmp.get(),
+                                                           kappa_u.get(), (int4*)h.conn.get(), h.kappa.get());
+    HIP_LAUNCH_CHECK();
+  }
+  // vertex -> element incidence
+  {
+    int64_t m = npe * ne;
+    DevBuf<uint32_t> key(m), key2(m);
+    DevBuf<int32_t> val(m);
+    v2e_keys_kernel<<<grid_for(ne, 256), 256, 0, s>>>(ne, npe, (const int4*)h.conn.get(), key.get(), val.get());
+    HIP_LAUNCH_CHECK();
+    h.v2e.resize(m);
+    dev_sort_pairs_u32_i32(key.get(), key2.get(), val.get(), h.v2e.get(), (size_t)m, 32, s);
+    h.v2e_ptr.resize(nv + 1);
+    lower_bound_kernel<<<grid_for(nv + 1, 256), 256, 0, s>>>(nv, key2.get(), m, h.v2e_ptr.get());
+    HIP_LAUNCH_CHECK();
+  }
+  // pattern
+  {
+    DevBuf<int32_t> cnt(nv);
+    pattern_rows_kernel<false><<<grid_for(nv, kRowBlock), kRowBlock, 0, s>>>(
+        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+    // long rows
+    DevBuf<int32_t> long_rows(nv);
+    DevBuf<int64_t> cap(nv + 1);
+    DevBuf<unsigned> nlong_d(1);
+    HIP_CHECK(hipMemsetAsync(nlong_d.get(), 0, sizeof(unsigned), s));
+    collect_long_rows_kernel<<<grid_for(nv, 256), 256, 0, s>>>(nv, cnt.get(), h.v2e_ptr.get(), npe,
+                                                               long_rows.get(), cap.get(), nlong_d.get());
+    HIP_LAUNCH_CHECK();
+    unsigned nlong = 0;
+    HIP_CHECK(hipMemcpyAsync(&nlong, nlong_d.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    DevBuf<int64_t> off;
+    DevBuf<int> scratch;
+    if (nlong) {
+      // deterministic processing order of long rows does not matter: each is independent
+      off.resize(nlong + 1);
+      int64_t tot = dev_exclusive_scan_i64(cap.get(), off.get(), nlong, s);
+      scratch.resize(tot);
+      pattern_rows_global_kernel<false><<<grid_for(nlong, 64), 64, 0, s>>>(
+          nlong, long_rows.get(), off.get(), npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(),
+          scratch.get(), cnt.get(), nullptr, nullptr);
+      HIP_LAUNCH_CHECK();
+    }
+    h.A.rp.resize(nv + 1);
+    int64_t nnz = dev_exclusive_scan_i32(cnt.get(), h.A.rp.get(), nv, s);
+    if (nnz >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "matrix has 2^31 or more nonzeros");
+    h.A.n = h.A.m = nv;
+    h.A.nnz = nnz;
+    h.A.ci.resize(nnz);
+    pattern_rows_kernel<true><<<grid_for(nv, kRowBlock), kRowBlock, 0, s>>>(
+        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), h.A.rp.get(), h.A.ci.get());
+    HIP_LAUNCH_CHECK();
+    if (nlong) {
+      pattern_rows_global_kernel<true><<<grid_for(nlong, 64), 64, 0, s>>>(
+          nlong, long_rows.get(), off.get(), npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(),
+          scratch.get(), cnt.get(), h.A.rp.get(), h.A.ci.get());
+      HIP_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  h.A.v64.resize(h.A.nnz);
+  h.has_mesh = true;
+  assemble_values(h);
+}
+
+void assemble_values(Handle& h) {
+  hipStream_t s = h.stream;
+  const int npe = h.dim + 1;
+  ProfScope ps(h, "assembly", (double)h.ne * (16 + 8) + (double)h.nv * 32 + (double)h.A.nnz * 12 +
+                                  (double)h.ne * npe * 4);
+  assemble_rows_kernel<<<grid_for(h.nv, kRowBlock), kRowBlock, 0, s>>>(
+      h.nv, h.dim, npe, (h.flags & 1) ? 1 : 0, h.mass_coeff, h.v2e_ptr.get(), h.v2e.get(),
+      (const int4*)h.conn.get(), (const double4*)h.coords.get(), h.kappa.get(), h.A.rp.get(), h.A.ci.get(),
+      h.A.v64.get());
+  HIP_LAUNCH_CHECK();
+}
+
+void element_gradient(Handle& h, const double* a_int, const double* u_int, double* g_user) {
+  ProfScope ps(h, "element_grad", (double)h.ne * (16 + 4 + 8) + (double)h.nv * (32 + 16));
+  element_gradient_kernel<<<grid_for(h.ne, 256), 256, 0, h.stream>>>(
+      h.ne, h.dim, h.dim + 1, (const int4*)h.conn.get(), (const double4*)h.coords.get(), h.eperm.get(), a_int,
+      u_int, g_user);
+  HIP_LAUNCH_CHECK();
+}
+
+void set_kappa_user(Handle& h, const double* kappa_user_dev) {
+  kappa_gather_kernel<<<grid_for(h.ne, 256), 256, 0, h.stream>>>(h.ne, h.eperm.get(), kappa_user_dev, h.kappa.get());
+  HIP_LAUNCH_CHECK();
+}
+
+void gather_user_to_int(Handle& h, const double* u, double* i, int k) {
+  int64_t n = h.A.n;
+  gather_kernel<<<grid_for(n * k, 256), 256, 0, h.stream>>>(n, k, h.perm.get(), u, i);
+  HIP_LAUNCH_CHECK();
+}
+
+void scatter_int_to_user(Handle& h, const double* i, double* u, int k) {
+  int64_t n = h.A.n;
+  scatter_kernel<<<grid_for(n * k, 256), 256, 0, h.stream>>>(n, k, h.perm.get(), i, u);
+  HIP_LAUNCH_CHECK();
+}
+
+}  // namespace fem

@@ -1,0 +1,170 @@
+// Internal data structures shared by the translation units of libfem_mi355x.
+//
+// Device layout (all in the library's internal vertex numbering, a Morton
+// order of the vertex coordinates, so that rows that are close in the mesh are
+// close in memory):
+//   coords   : nv x double4 (x, y, z, 0)                         32 B / vertex
+//   conn     : ne x int4 (tets) / int4 with w = -1 (triangles)    16 B / element
+//   kappa    : ne doubles (element order sorted by lowest vertex)
+//   v2e      : CSR vertex -> (element*4 + local corner), ascending element
+//   A        : CSR int32 row_ptr / int32 cols (ascending) / FP64 values,
+//              plus an FP32 copy of every level in mixed precision.
+#pragma once
+
+#include "../../include/fem_mi355x.h"
+#include "common.hpp"
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace fem {
+
+// Compressed sparse rows on the device.
+struct Csr {
+  int64_t n = 0;    // rows
+  int64_t m = 0;    // columns
+  int64_t nnz = 0;
+  DevBuf<int32_t> rp;  // n + 1
+  DevBuf<int32_t> ci;  // nnz
+  DevBuf<double> v64;  // nnz
+  DevBuf<float> v32;   // nnz (mixed precision only)
+};
+
+struct Level {
+  Csr A;
+  DevBuf<double> dinv64;
+  DevBuf<float> dinv32;
+  Csr P;                  // n x nc prolongator
+  Csr R;                  // nc x n restriction = P^T
+  DevBuf<int32_t> agg;    // n, aggregate of each row
+  int64_t nc = 0;
+  double rho = 2.0;       // spectral radius estimate of D^-1 A
+  // V-cycle work vectors (one RHS column block of width `wk`)
+  int wk = 0;
+  DevBuf<double> x64, t64, b64, r64, d64;
+  DevBuf<float> x32, t32, b32, r32, d32;
+};
+
+struct Hierarchy {
+  std::vector<std::unique_ptr<Level>> levels;  // levels[0].A aliases nothing: its own copy
+  int coarse = -1;                             // index of the coarsest level
+  int64_t nc = 0;                              // coarsest size
+  DevBuf<double> cinv64;                       // nc x nc dense inverse
+  DevBuf<float> cinv32;
+  double grid_complexity = 0, operator_complexity = 0;
+  double setup_ms = 0;
+  int precision = 0;
+  bool built = false;
+};
+
+struct KernelStat {
+  int64_t launches = 0;
+  double total_ms = 0;
+  double bytes = 0;
+};
+
+struct Handle {
+  std::string err;
+  fem_params prm{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+
+  // mesh (host copies, user numbering; kept for IO and neighbour queries)
+  int dim = 3;
+  int64_t nv = 0, ne = 0;
+  std::vector<double> h_xyz;     // nv x 3
+  std::vector<int32_t> h_conn;   // ne x (dim+1)
+  std::vector<double> h_kappa;   // ne
+  double mass_coeff = 1.0;
+  int flags = 0;
+
+  // device mesh, internal numbering
+  DevBuf<double> coords;     // nv x 4
+  DevBuf<int32_t> conn;      // ne x 4
+  DevBuf<double> kappa;      // ne
+  DevBuf<int32_t> perm;      // internal -> user vertex
+  DevBuf<int32_t> iperm;     // user -> internal vertex
+  DevBuf<int32_t> eperm;     // internal -> user element
+  DevBuf<int32_t> v2e_ptr;   // nv + 1
+  DevBuf<int32_t> v2e;       // 4 ne (or 3 ne)
+  bool has_mesh = false;
+  bool matrix_from_file = false;
+
+  // level-0 operator (FP64, internal numbering)
+  Csr A;
+  std::vector<int64_t> mesh_pattern_rp;  // user-numbered mesh pattern (host)
+  std::vector<int32_t> mesh_pattern_ci;
+
+  Hierarchy H;
+
+  // Krylov work (FP64), capacity for `kcap` right-hand sides
+  int kcap = 0;
+  DevBuf<double> kx, kr, kp, kq, kz, kb;
+  DevBuf<double> partials;        // reduction partials
+  DevBuf<unsigned> counters;      // last-block counters
+  DevBuf<double> scal;            // device scalars
+  double* h_scal = nullptr;       // pinned mirror of scal
+  DevBuf<double> ubuf, ubuf2;     // user-order staging (nv * k)
+
+  bool profiling = false;
+  std::map<std::string, KernelStat> stats;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// ---- primitives (primitives.hip) ----
+void dev_sort_pairs_u64_i32(const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout,
+                            size_t n, int end_bit, hipStream_t s);
+void dev_sort_pairs_u32_i32(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout,
+                            size_t n, int end_bit, hipStream_t s);
+// out[0] = 0, out[i] = sum(in[0..i)), out has n + 1 entries; returns total
+int64_t dev_exclusive_scan_i32(const int32_t* in, int32_t* out, size_t n, hipStream_t s);
+int64_t dev_exclusive_scan_i64(const int64_t* in, int64_t* out, size_t n, hipStream_t s);
+void dev_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t s);
+void dev_fill_f64(double* p, double v, size_t n, hipStream_t s);
+void dev_fill_f32(float* p, float v, size_t n, hipStream_t s);
+void dev_iota_i32(int32_t* p, size_t n, hipStream_t s);
+void dev_f64_to_f32(const double* in, float* out, size_t n, hipStream_t s);
+
+// ---- mesh, pattern and assembly (assembly.hip) ----
+void build_mesh_device(Handle& h);          // reorder + v2e + pattern + values
+void assemble_values(Handle& h);            // values only, pattern fixed
+void element_gradient(Handle& h, const double* a_int, const double* u_int, double* g_user);
+void gather_user_to_int(Handle& h, const double* u, double* i, int k);   // i[r] = u[perm[r]]
+void scatter_int_to_user(Handle& h, const double* i, double* u, int k);  // u[perm[r]] = i[r]
+
+// ---- sparse building blocks (sparse.hip) ----
+// C = L * Rm with optional column map on Rm's columns and optional removal of
+// the diagonal; pattern only (sorted unique columns).
+void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int64_t r_rows,
+                     const int32_t* colmap, int64_t ncols, bool drop_diag, Csr& C, hipStream_t s);
+// values of C = L * Rm (FP64) for the pattern found by spgemm_symbolic
+void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
+void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
+void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
+
+// ---- AMG setup (amg_setup.hip) ----
+void build_hierarchy(Handle& h);
+void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth, int min_size,
+                     int rule, unsigned long long seed, int32_t* agg_out, int64_t* n_agg,
+                     hipStream_t s);
+
+// ---- solve (solve.hip) ----
+void ensure_krylov(Handle& h, int k);
+int pcg_solve(Handle& h, int k, fem_solve_info* info);   // uses h.kb (rhs), h.kx (x0 / out), internal order
+void apply_vcycle_fp64(Handle& h, const double* r, double* z);  // level-0 preconditioner, 1 column
+void apply_operator_int(Handle& h, const double* x, double* y, int k);
+
+// profiling helper: records kernel time into h.stats when enabled
+struct ProfScope {
+  Handle& h;
+  const char* fam;
+  double bytes;
+  bool on;
+  ProfScope(Handle& hh, const char* f, double b);
+  ~ProfScope();
+};
+
+}  // namespace fem

@@ -1,0 +1,155 @@
+// Device kernels of the solve phase: CSR SpMV / SpMM with fused smoother
+// epilogues, reductions with a last-block finish, and the dense coarse solve.
+//
+// Layout: vectors with k right-hand sides are row-major (n x k), so the k
+// values of one vertex are contiguous and a gather x[col*k + c] by k lanes is
+// one coalesced access.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fem {
+
+constexpr int kNT = 256;  // threads per block of the solve kernels
+
+enum SpmvMode : int {
+  kAssign = 0,   // y = A x
+  kResid = 1,    // y = b - A x
+  kAdd = 2,      // y += A x
+  kJacobi = 3,   // xn = x + c1 * dinv * (b - A x)
+  kCheb = 4,     // r = dinv (b - A x); d = c1 d + c2 r; xn = x + d
+};
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor_t(T v, int mask, int width) {
+  return __shfl_xor(v, mask, width);
+}
+
+// One epilogue for all modes; `acc` = (A x)_row for column c.
+template <int MODE, typename T, typename V>
+__device__ __forceinline__ void spmv_epilogue(int64_t idx, int64_t row, T acc, T* __restrict__ y,
+                                              const T* __restrict__ x, const T* __restrict__ b,
+                                              const V* __restrict__ dinv, T* __restrict__ d,
+                                              T* __restrict__ xn, T c1, T c2) {
+  if (MODE == kAssign) {
+    y[idx] = acc;
+  } else if (MODE == kResid) {
+    y[idx] = b[idx] - acc;
+  } else if (MODE == kAdd) {
+    y[idx] += acc;
+  } else if (MODE == kJacobi) {
+    xn[idx] = x[idx] + c1 * (T)dinv[row] * (b[idx] - acc);
+  } else {  // kCheb
+    T r = (T)dinv[row] * (b[idx] - acc);
+    T dn = (c1 != T(0) ? c1 * d[idx] : T(0)) + c2 * r;
+    d[idx] = dn;
+    xn[idx] = x[idx] + dn;
+  }
+}
+
+// Single right-hand side: G lanes per row, strided over the row's entries,
+// reduced by a butterfly (fixed order).
+template <typename V, typename T, int G, int MODE>
+__global__ void __launch_bounds__(kNT) csr_vec_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const V* __restrict__ val,
+                                                    const T* __restrict__ x, T* __restrict__ y,
+                                                    const T* __restrict__ b, const V* __restrict__ dinv,
+                                                    T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
+  const int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (row >= n) return;
+  const int s = rp[row], e = rp[row + 1];
+  T acc = T(0);
+  for (int k = s + lane; k < e; k += G) acc += (T)val[k] * x[ci[k]];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += shfl_xor_t(acc, off, G);
+  if (lane == 0) spmv_epilogue<MODE, T, V>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
+}
+
+// k right-hand sides: KP lanes per row (KP = power of two >= k), lane c owns
+// column c and walks the row sequentially.
+template <typename V, typename T, int KP, int MODE>
+__global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ ci, const V* __restrict__ val,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      const T* __restrict__ b, const V* __restrict__ dinv,
+                                                      T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
+  const int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP;
+  const int c = threadIdx.x & (KP - 1);
+  if (row >= n || c >= k) return;
+  const int s = rp[row], e = rp[row + 1];
+  T acc = T(0);
+  for (int q = s; q < e; ++q) acc += (T)val[q] * x[(int64_t)ci[q] * k + c];
+  spmv_epilogue<MODE, T, V>(row * k + c, row, acc, y, x, b, dinv, d, xn, c1, c2);
+}
+
+// ---- reductions ---------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of v over the block (fixed order); result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* s_buf) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_buf[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kNT / 64; ++i) r += s_buf[i];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double load_agent(const double* p) {
+  unsigned long long u = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+
+// Per-column block partials for row-major (n x k) data: thread (row, c) with
+// c = tid % KP.  Writes partials[blockIdx.x * k + c] for c < k.
+template <int KP>
+__device__ __forceinline__ void block_col_partials(double v, int k, double* s_buf, double* partials) {
+  if (KP == 1) {
+    double r = block_sum(v, s_buf);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+    return;
+  }
+  s_buf[threadIdx.x] = v;
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double s = 0.0;
+    for (int j = threadIdx.x; j < kNT; j += KP) s += s_buf[j];
+    partials[(int64_t)blockIdx.x * k + threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Returns true in every thread of the block that finished last; that block
+// may then read all partials (agent-scope loads) and must reset the counter.
+__device__ __forceinline__ bool last_block(unsigned* counter, unsigned* s_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = atomicAdd(counter, 1u);
+    *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  bool last = (*s_flag != 0u);
+  if (last) __threadfence();
+  return last;
+}
+
+// total over nb partials for column c (called by the whole last block)
+__device__ __forceinline__ double finish_col(const double* partials, int nb, int k, int c, double* s_buf) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kNT) v += load_agent(partials + (int64_t)b * k + c);
+  return block_sum(v, s_buf);
+}
+
+}  // namespace fem

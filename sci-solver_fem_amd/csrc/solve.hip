@@ -1,0 +1,709 @@
+// V-cycle and preconditioned CG.
+//
+// Reference path being replaced: CG_Flex_Cycle (src/core/cuda/cgcycle.cu:7-69),
+// AMG_Level::cycle / cycle_level0 (src/core/cuda/amg_level.cu:22-128) and the
+// smoother kernels of src/core/cuda/gauss_seidel.cu (preRRRFullSymmetric
+// :2018-2261, postPCRFullSymmetric :4409-4662).
+//
+// Differences by design (results are compared on the solution, which the
+// preconditioner does not change):
+//  * the V-cycle is symmetric (same polynomial smoother before and after the
+//    coarse correction), so plain CG is valid for it;
+//  * the outer residual stays FP64 and is never overwritten by the V-cycle;
+//  * the stop test is re-checked on the true residual b - A x;
+//  * scalars (alpha, beta, norms) stay on the device; the host reads one
+//    vector of squared residual norms per iteration.
+#include <cmath>
+#include <cstring>
+
+#include "fem_internal.hpp"
+#include "kernels.hpp"
+
+namespace fem {
+
+namespace {
+
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_ACT, S_RZN, S_TRUE, S_COUNT };
+enum { C_PQ = 0, C_RR, C_DOT, C_COUNT };  // last-block counters
+
+int pow2_at_least(int k) {
+  int p = 1;
+  while (p < k) p <<= 1;
+  return p;
+}
+
+int lanes_for(const Csr& A) {
+  double avg = A.n ? (double)A.nnz / (double)A.n : 1.0;
+  if (avg <= 6) return 4;
+  if (avg <= 12) return 8;
+  if (avg <= 24) return 16;
+  if (avg <= 48) return 32;
+  return 64;
+}
+
+const Csr& level_A(const Handle& h, int l) { return l == 0 ? h.A : h.H.levels[l]->A; }
+
+template <typename V>
+const V* vals_of(const Csr& A);
+template <>
+const double* vals_of<double>(const Csr& A) { return A.v64.get(); }
+template <>
+const float* vals_of<float>(const Csr& A) { return A.v32.get(); }
+
+template <typename V>
+const V* dinv_of(const Level& L);
+template <>
+const double* dinv_of<double>(const Level& L) { return L.dinv64.get(); }
+template <>
+const float* dinv_of<float>(const Level& L) { return L.dinv32.get(); }
+
+template <typename T>
+struct LV;
+template <>
+struct LV<double> {
+  static double* x(Level& L) { return L.x64.get(); }
+  static double* t(Level& L) { return L.t64.get(); }
+  static double* b(Level& L) { return L.b64.get(); }
+  static double* r(Level& L) { return L.r64.get(); }
+  static double* d(Level& L) { return L.d64.get(); }
+  static const double* cinv(const Hierarchy& H) { return H.cinv64.get(); }
+};
+template <>
+struct LV<float> {
+  static float* x(Level& L) { return L.x32.get(); }
+  static float* t(Level& L) { return L.t32.get(); }
+  static float* b(Level& L) { return L.b32.get(); }
+  static float* r(Level& L) { return L.r32.get(); }
+  static float* d(Level& L) { return L.d32.get(); }
+  static const float* cinv(const Hierarchy& H) { return H.cinv32.get(); }
+};
+
+template <typename V, typename T, int MODE, int G>
+void launch_vec(const Csr& A, const V* val, const T* x, T* y, const T* b, const V* dinv, T* d, T* xn, T c1, T c2,
+                hipStream_t s) {
+  unsigned grid = grid_for(A.n * G, kNT);
+  csr_vec_kernel<V, T, G, MODE><<<grid, kNT, 0, s>>>(A.n, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn, c1, c2);
+}
+
+template <typename V, typename T, int MODE, int KP>
+void launch_multi(const Csr& A, int k, const V* val, const T* x, T* y, const T* b, const V* dinv, T* d, T* xn, T c1,
+                  T c2, hipStream_t s) {
+  unsigned grid = grid_for(A.n * KP, kNT);
+  csr_multi_kernel<V, T, KP, MODE><<<grid, kNT, 0, s>>>(A.n, k, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn,
+                                                       c1, c2);
+}
+
+// y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.
+template <typename V, typename T, int MODE>
+void spmv_op(const Csr& A, int k, const V* val, const T* x, T* y, const T* b, const V* dinv, T* d, T* xn, T c1, T c2,
+             hipStream_t s) {
+  if (A.n == 0) return;
+  if (k == 1) {
+    switch (lanes_for(A)) {
+      case 4: launch_vec<V, T, MODE, 4>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_vec<V, T, MODE, 8>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_vec<V, T, MODE, 16>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 32: launch_vec<V, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_vec<V, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+    }
+  } else {
+    switch (pow2_at_least(k)) {
+      case 2: launch_multi<V, T, MODE, 2>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_multi<V, T, MODE, 4>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_multi<V, T, MODE, 8>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_multi<V, T, MODE, 16>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 32: launch_multi<V, T, MODE, 32>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_multi<V, T, MODE, 64>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+    }
+  }
+  HIP_LAUNCH_CHECK();
+}
+
+// ---- elementwise kernels ------------------------------------------------------
+
+// x = c1 * dinv * b (first smoothing step from a zero guess); d = x if d != null
+template <typename V, typename Tb, typename T>
+__global__ void scale_dinv_kernel(int64_t n, int k, const V* __restrict__ dinv, const Tb* __restrict__ b,
+                                  T* __restrict__ x, T* __restrict__ d, T c1) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  int64_t i = t / k;
+  T v = c1 * (T)dinv[i] * (T)b[t];
+  x[t] = v;
+  if (d) d[t] = v;
+}
+
+template <typename Tin, typename Tout>
+__global__ void convert_kernel(int64_t m, const Tin* __restrict__ a, Tout* __restrict__ b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) b[t] = (Tout)a[t];
+}
+
+// dense coarse solve x = C b, C row-major nc x nc; one wave per row
+template <typename T>
+__global__ void __launch_bounds__(64) coarse_kernel(int64_t nc, int k, const T* __restrict__ C, const T* __restrict__ b,
+                                                   T* __restrict__ x) {
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int c = 0; c < k; ++c) {
+    double acc = 0.0;
+    for (int64_t j = lane; j < nc; j += 64) acc += (double)C[i * nc + j] * (double)b[j * k + c];
+    acc = wave_sum(acc);
+    if (lane == 0) x[i * k + c] = (T)acc;
+  }
+}
+
+// ---- PCG vector kernels (FP64, k columns, row-major) --------------------------
+
+// q = A p and per-column p.q; last block: alpha = act ? rz / pq : 0
+template <int G>
+__global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ci, const double* __restrict__ val,
+                                                         const double* __restrict__ p, double* __restrict__ q,
+                                                         double* __restrict__ partials, unsigned* __restrict__ counter,
+                                                         double* __restrict__ scal) {
+  __shared__ double s_buf[kNT];
+  __shared__ unsigned s_flag;
+  double part = 0.0;
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / G);
+  for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / G; row < n; row += rows_per_grid) {
+    const int s = rp[row], e = rp[row + 1];
+    double acc = 0.0;
+    for (int j = s + lane; j < e; j += G) acc += val[j] * p[ci[j]];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (lane == 0) {
+      q[row] = acc;
+      part += acc * p[row];
+    }
+  }
+  double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  if (last_block(counter, &s_flag)) {
+    double pq = finish_col(partials, gridDim.x, 1, 0, s_buf);
+    if (threadIdx.x == 0) {
+      scal[S_PQ] = pq;
+      scal[S_ALPHA] = (scal[S_ACT] != 0.0 && pq != 0.0) ? scal[S_RZ] / pq : 0.0;
+      *counter = 0u;
+    }
+  }
+}
+
+template <int KP>
+__global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ci,
+                                                         const double* __restrict__ val, const double* __restrict__ p,
+                                                         double* __restrict__ q, double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter, double* __restrict__ scal) {
+  __shared__ double s_buf[kNT];
+  __shared__ unsigned s_flag;
+  double part = 0.0;
+  const int c = threadIdx.x & (KP - 1);
+  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
+  if (c < k) {
+    for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid) {
+      const int s = rp[row], e = rp[row + 1];
+      double acc = 0.0;
+      for (int j = s; j < e; ++j) acc += val[j] * p[(int64_t)ci[j] * k + c];
+      q[row * k + c] = acc;
+      part += acc * p[row * k + c];
+    }
+  }
+  block_col_partials<KP>(part, k, s_buf, partials);
+  if (last_block(counter, &s_flag)) {
+    for (int cc = 0; cc < k; ++cc) {
+      double pq = finish_col(partials, gridDim.x, k, cc, s_buf);
+      if (threadIdx.x == 0) {
+        scal[S_PQ * k + cc] = pq;
+        scal[S_ALPHA * k + cc] = (scal[S_ACT * k + cc] != 0.0 && pq != 0.0) ? scal[S_RZ * k + cc] / pq : 0.0;
+      }
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+}
+
+// x += alpha p; r -= alpha q; rr = r.r; last block: act &= rr > tol2 * bb
+template <int KP>
+__global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, double* __restrict__ x,
+                                                       double* __restrict__ r, const double* __restrict__ p,
+                                                       const double* __restrict__ q, double* __restrict__ partials,
+                                                       unsigned* __restrict__ counter, double* __restrict__ scal,
+                                                       double tol2) {
+  __shared__ double s_buf[kNT];
+  __shared__ unsigned s_flag;
+  double part = 0.0;
+  const int c = threadIdx.x & (KP - 1);
+  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
+  if (c < k) {
+    const double a = scal[S_ALPHA * k + c];
+    for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid) {
+      int64_t t = row * k + c;
+      x[t] += a * p[t];
+      double rv = r[t] - a * q[t];
+      r[t] = rv;
+      part += rv * rv;
+    }
+  }
+  block_col_partials<KP>(part, k, s_buf, partials);
+  if (last_block(counter, &s_flag)) {
+    for (int cc = 0; cc < k; ++cc) {
+      double rr = finish_col(partials, gridDim.x, k, cc, s_buf);
+      if (threadIdx.x == 0) {
+        scal[S_RR * k + cc] = rr;
+        bool act = scal[S_ACT * k + cc] != 0.0 && rr > tol2 * scal[S_BB * k + cc];
+        scal[S_ACT * k + cc] = act ? 1.0 : 0.0;
+      }
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+}
+
+// per-column a.b; finish: 0 -> scal[slot] = dot; 1 -> beta = act ? dot/rz : 0, rz = dot
+template <int KP, int FIN>
+__global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const double* __restrict__ a,
+                                                    const double* __restrict__ b, double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter, double* __restrict__ scal,
+                                                    int slot) {
+  __shared__ double s_buf[kNT];
+  __shared__ unsigned s_flag;
+  double part = 0.0;
+  const int c = threadIdx.x & (KP - 1);
+  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
+  if (c < k)
+    for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid)
+      part += a[row * k + c] * b[row * k + c];
+  block_col_partials<KP>(part, k, s_buf, partials);
+  if (last_block(counter, &s_flag)) {
+    for (int cc = 0; cc < k; ++cc) {
+      double v = finish_col(partials, gridDim.x, k, cc, s_buf);
+      if (threadIdx.x == 0) {
+        if (FIN == 0) {
+          scal[slot * k + cc] = v;
+        } else {
+          double rz = scal[S_RZ * k + cc];
+          scal[S_BETA * k + cc] = (scal[S_ACT * k + cc] != 0.0 && rz != 0.0) ? v / rz : 0.0;
+          scal[S_RZ * k + cc] = v;
+        }
+      }
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+}
+
+// p = z + beta p
+__global__ void pcg_p_kernel(int64_t m, int k, const double* __restrict__ z, double* __restrict__ p,
+                             const double* __restrict__ scal) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int c = (int)(t % k);
+  p[t] = z[t] + scal[S_BETA * k + c] * p[t];
+}
+
+__global__ void copy_kernel(int64_t m, const double* __restrict__ a, double* __restrict__ b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) b[t] = a[t];
+}
+
+unsigned vec_grid(int64_t n, int kp) {
+  int64_t g = (n * kp + kNT - 1) / kNT;
+  if (g > 1024) g = 1024;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ---- V-cycle ---------------------------------------------------------------------
+
+struct Smoother {
+  int type;        // 0 Jacobi, 1 Chebyshev
+  double omega;    // Jacobi weight
+  double lmin, lmax;
+};
+
+Smoother smoother_for(const Handle& h, const Level& L) {
+  Smoother S;
+  S.type = h.prm.smootherType;
+  S.omega = h.prm.smootherWeight * (4.0 / 3.0) / L.rho;
+  S.lmax = 1.1 * L.rho;
+  S.lmin = S.lmax / 30.0;
+  return S;
+}
+
+// nu steps of the smoother from x = 0 (zero == true) or from the given x.
+// Buffers x and t ping-pong; returns the buffer that holds the result.
+template <typename T>
+T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, bool zero, int k) {
+  Level& L = *h.H.levels[l];
+  hipStream_t s = h.stream;
+  const T* val = vals_of<T>(A);
+  const T* dinv = dinv_of<T>(L);
+  Smoother S = smoother_for(h, L);
+  const bool prof = h.profiling && l == 0;
+  const double sweep_bytes = (double)A.nnz * (sizeof(T) + 4) + 4.0 * (A.n + 1) +
+                             (double)A.n * (sizeof(T) * (S.type ? 6.0 * k : 3.0 * k) + sizeof(T));
+  if (nu <= 0) {
+    if (zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(T) * A.n * k, s));
+    return x;
+  }
+  T* cur = x;
+  T* nxt = t;
+  int step = 0;
+  if (S.type == 0) {
+    const T w = (T)S.omega;
+    if (zero) {
+      scale_dinv_kernel<T, T, T><<<grid_for(A.n * k, 256), 256, 0, s>>>(A.n, k, dinv, b, cur, (T*)nullptr, w);
+      HIP_LAUNCH_CHECK();
+      step = 1;
+    }
+    for (; step < nu; ++step) {
+      ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
+      spmv_op<T, T, kJacobi>(A, k, val, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
+      T* tmp = cur; cur = nxt; nxt = tmp;
+    }
+    return cur;
+  }
+  // Chebyshev on D^-1 A over [lmin, lmax]
+  const double theta = 0.5 * (S.lmax + S.lmin), delta = 0.5 * (S.lmax - S.lmin);
+  const double sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  if (zero) {
+    scale_dinv_kernel<T, T, T><<<grid_for(A.n * k, 256), 256, 0, s>>>(A.n, k, dinv, b, cur, d, (T)(1.0 / theta));
+    HIP_LAUNCH_CHECK();
+    step = 1;
+  }
+  for (; step < nu; ++step) {
+    double c1, c2;
+    if (step == 0) {
+      c1 = 0.0;
+      c2 = 1.0 / theta;
+    } else {
+      double rho_new = 1.0 / (2.0 * sigma - rho_old);
+      c1 = rho_new * rho_old;
+      c2 = 2.0 * rho_new / delta;
+      rho_old = rho_new;
+    }
+    ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
+    spmv_op<T, T, kCheb>(A, k, val, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
+    T* tmp = cur; cur = nxt; nxt = tmp;
+  }
+  return cur;
+}
+
+// V-cycle below level l for the right-hand side b (level l vectors, k columns);
+// returns the buffer holding the correction.
+template <typename T>
+T* vcycle(Handle& h, int l, const T* b, int k) {
+  Hierarchy& H = h.H;
+  Level& L = *H.levels[l];
+  hipStream_t s = h.stream;
+  const Csr& A = level_A(h, l);
+  if (l == H.coarse) {
+    coarse_kernel<T><<<(unsigned)H.nc, 64, 0, s>>>(H.nc, k, LV<T>::cinv(H), b, LV<T>::x(L));
+    HIP_LAUNCH_CHECK();
+    return LV<T>::x(L);
+  }
+  T* x = LV<T>::x(L);
+  T* t = LV<T>::t(L);
+  T* r = LV<T>::r(L);
+  T* d = LV<T>::d(L);
+  const int nu_pre = h.prm.preInnerIters;
+  const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
+  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k);
+  T* oth = (cur == x) ? t : x;
+  {
+    ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", (double)A.nnz * (sizeof(T) + 4) + 4.0 * (A.n + 1) +
+                                                                 3.0 * A.n * k * sizeof(T));
+    spmv_op<T, T, kResid>(A, k, vals_of<T>(A), cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+  }
+  Level& N = *H.levels[l + 1];
+  T* bc = LV<T>::b(N);
+  spmv_op<T, T, kAssign>(L.R, k, vals_of<T>(L.R), r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+  T* xc = vcycle<T>(h, l + 1, bc, k);
+  spmv_op<T, T, kAdd>(L.P, k, vals_of<T>(L.P), xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+  cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k);
+  return cur;
+}
+
+// z = M r at level 0 (FP64 in and out); k columns
+void precondition(Handle& h, const double* r, double* z, int k) {
+  hipStream_t s = h.stream;
+  Level& L0 = *h.H.levels[0];
+  const int64_t m = h.A.n * k;
+  if (h.H.precision == 0) {
+    double* res = vcycle<double>(h, 0, r, k);
+    copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
+    HIP_LAUNCH_CHECK();
+  } else {
+    convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
+    HIP_LAUNCH_CHECK();
+    float* res = vcycle<float>(h, 0, L0.b32.get(), k);
+    convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
+    HIP_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace
+
+ProfScope::ProfScope(Handle& hh, const char* f, double b) : h(hh), fam(f), bytes(b), on(hh.profiling && f && f[0]) {
+  if (on) HIP_CHECK(hipEventRecord(h.ev0, h.stream));
+}
+ProfScope::~ProfScope() {
+  if (!on) return;
+  if (hipEventRecord(h.ev1, h.stream) != hipSuccess) return;
+  if (hipEventSynchronize(h.ev1) != hipSuccess) return;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h.ev0, h.ev1) != hipSuccess) return;
+  KernelStat& st = h.stats[fam];
+  st.launches += 1;
+  st.total_ms += ms;
+  st.bytes = bytes;
+}
+
+void spmv64(const Csr& A, const double* x, double* y, hipStream_t s) {
+  spmv_op<double, double, kAssign>(A, 1, A.v64.get(), x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0,
+                                   0.0, s);
+}
+
+void apply_operator_int(Handle& h, const double* x, double* y, int k) {
+  spmv_op<double, double, kAssign>(h.A, k, h.A.v64.get(), x, y, nullptr, (const double*)nullptr, nullptr, nullptr,
+                                   0.0, 0.0, h.stream);
+}
+
+void ensure_krylov(Handle& h, int k) {
+  const int64_t n = h.A.n;
+  if (h.kcap < k) {
+    h.kx.resize(n * k); h.kr.resize(n * k); h.kp.resize(n * k); h.kq.resize(n * k);
+    h.kz.resize(n * k); h.kb.resize(n * k);
+    h.ubuf.resize(n * k); h.ubuf2.resize(n * k);
+    h.partials.resize((size_t)4096 * 64);
+    h.counters.resize(C_COUNT);
+    HIP_CHECK(hipMemsetAsync(h.counters.get(), 0, sizeof(unsigned) * C_COUNT, h.stream));
+    h.scal.resize((size_t)S_COUNT * 64);
+    if (!h.h_scal) HIP_CHECK(hipHostMalloc((void**)&h.h_scal, sizeof(double) * S_COUNT * 64, hipHostMallocDefault));
+    h.kcap = k;
+  }
+  for (auto& Lp : h.H.levels) {
+    Level& L = *Lp;
+    if (L.wk >= k) continue;
+    const int64_t nl = (&L == h.H.levels[0].get()) ? h.A.n : L.A.n;
+    if (h.H.precision == 0) {
+      L.x64.resize(nl * k); L.t64.resize(nl * k); L.b64.resize(nl * k); L.r64.resize(nl * k); L.d64.resize(nl * k);
+    } else {
+      L.x32.resize(nl * k); L.t32.resize(nl * k); L.b32.resize(nl * k); L.r32.resize(nl * k); L.d32.resize(nl * k);
+    }
+    L.wk = k;
+  }
+}
+
+void apply_vcycle_fp64(Handle& h, const double* r, double* z) { precondition(h, r, z, 1); }
+
+namespace {
+
+template <int KP>
+void launch_dot(Handle& h, int k, const double* a, const double* b, int fin, int slot) {
+  const int64_t n = h.A.n;
+  unsigned g = vec_grid(n, KP);
+  if (fin == 0)
+    pcg_dot_kernel<KP, 0><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get(), h.counters.get() + C_DOT,
+                                                   h.scal.get(), slot);
+  else
+    pcg_dot_kernel<KP, 1><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get(), h.counters.get() + C_DOT,
+                                                   h.scal.get(), slot);
+  HIP_LAUNCH_CHECK();
+}
+
+void dot(Handle& h, int k, const double* a, const double* b, int fin, int slot) {
+  switch (pow2_at_least(k)) {
+    case 1: launch_dot<1>(h, k, a, b, fin, slot); break;
+    case 2: launch_dot<2>(h, k, a, b, fin, slot); break;
+    case 4: launch_dot<4>(h, k, a, b, fin, slot); break;
+    case 8: launch_dot<8>(h, k, a, b, fin, slot); break;
+    case 16: launch_dot<16>(h, k, a, b, fin, slot); break;
+    case 32: launch_dot<32>(h, k, a, b, fin, slot); break;
+    default: launch_dot<64>(h, k, a, b, fin, slot); break;
+  }
+}
+
+template <int KP>
+void launch_update(Handle& h, int k, double tol2) {
+  const int64_t n = h.A.n;
+  pcg_update_kernel<KP><<<vec_grid(n, KP), kNT, 0, h.stream>>>(n, k, h.kx.get(), h.kr.get(), h.kp.get(), h.kq.get(),
+                                                               h.partials.get(), h.counters.get() + C_RR,
+                                                               h.scal.get(), tol2);
+  HIP_LAUNCH_CHECK();
+}
+
+void update(Handle& h, int k, double tol2) {
+  switch (pow2_at_least(k)) {
+    case 1: launch_update<1>(h, k, tol2); break;
+    case 2: launch_update<2>(h, k, tol2); break;
+    case 4: launch_update<4>(h, k, tol2); break;
+    case 8: launch_update<8>(h, k, tol2); break;
+    case 16: launch_update<16>(h, k, tol2); break;
+    case 32: launch_update<32>(h, k, tol2); break;
+    default: launch_update<64>(h, k, tol2); break;
+  }
+}
+
+template <int G>
+void launch_spmv_dot(Handle& h) {
+  const Csr& A = h.A;
+  int64_t g = (A.n * G + kNT - 1) / kNT;
+  if (g > 4096) g = 4096;
+  pcg_spmv_dot_kernel<G><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, A.rp.get(), A.ci.get(), A.v64.get(), h.kp.get(),
+                                                            h.kq.get(), h.partials.get(), h.counters.get() + C_PQ,
+                                                            h.scal.get());
+  HIP_LAUNCH_CHECK();
+}
+
+template <int KP>
+void launch_spmm_dot(Handle& h, int k) {
+  const Csr& A = h.A;
+  int64_t g = (A.n * KP + kNT - 1) / kNT;
+  if (g > 4096) g = 4096;
+  pcg_spmm_dot_kernel<KP><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, k, A.rp.get(), A.ci.get(), A.v64.get(),
+                                                             h.kp.get(), h.kq.get(), h.partials.get(),
+                                                             h.counters.get() + C_PQ, h.scal.get());
+  HIP_LAUNCH_CHECK();
+}
+
+void spmv_dot(Handle& h, int k) {
+  const Csr& A = h.A;
+  ProfScope ps(h, "pcg_spmv", (double)A.nnz * 12 + 4.0 * (A.n + 1) + 3.0 * 8 * A.n * k);
+  if (k == 1) {
+    switch (lanes_for(A)) {
+      case 4: launch_spmv_dot<4>(h); break;
+      case 8: launch_spmv_dot<8>(h); break;
+      case 16: launch_spmv_dot<16>(h); break;
+      case 32: launch_spmv_dot<32>(h); break;
+      default: launch_spmv_dot<64>(h); break;
+    }
+  } else {
+    switch (pow2_at_least(k)) {
+      case 2: launch_spmm_dot<2>(h, k); break;
+      case 4: launch_spmm_dot<4>(h, k); break;
+      case 8: launch_spmm_dot<8>(h, k); break;
+      case 16: launch_spmm_dot<16>(h, k); break;
+      case 32: launch_spmm_dot<32>(h, k); break;
+      default: launch_spmm_dot<64>(h, k); break;
+    }
+  }
+}
+
+void set_scal(Handle& h, int slot, int k, const double* v) {
+  HIP_CHECK(hipMemcpyAsync(h.scal.get() + slot * k, v, sizeof(double) * k, hipMemcpyHostToDevice, h.stream));
+}
+
+void get_scal(Handle& h, int slot, int k, double* v) {
+  HIP_CHECK(hipMemcpyAsync(h.h_scal, h.scal.get() + slot * k, sizeof(double) * k, hipMemcpyDeviceToHost, h.stream));
+  HIP_CHECK(hipStreamSynchronize(h.stream));
+  std::memcpy(v, h.h_scal, sizeof(double) * k);
+}
+
+}  // namespace
+
+// Solve A x = b for k columns held in h.kb / h.kx (internal numbering).
+int pcg_solve(Handle& h, int k, fem_solve_info* info) {
+  hipStream_t s = h.stream;
+  const int64_t n = h.A.n;
+  const int64_t m = n * k;
+  const double tol = h.prm.tolerance;
+  const double tol2 = tol * tol;
+  std::vector<double> bb(k), rr(k), act(k), one(k, 1.0);
+
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, s));
+
+  dot(h, k, h.kb.get(), h.kb.get(), 0, S_BB);
+  get_scal(h, S_BB, k, bb.data());
+
+  int iters = 0;
+  int status = FEM_OK;
+  auto true_residual = [&](std::vector<double>& rel) {
+    // kq = b - A x ; rel_c = ||kq_c|| / ||b_c||
+    spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kq.get(), h.kb.get(), (const double*)nullptr,
+                                    nullptr, nullptr, 0.0, 0.0, s);
+    dot(h, k, h.kq.get(), h.kq.get(), 0, S_TRUE);
+    std::vector<double> t(k);
+    get_scal(h, S_TRUE, k, t.data());
+    rel.resize(k);
+    for (int c = 0; c < k; ++c) rel[c] = bb[c] > 0 ? std::sqrt(t[c] / bb[c]) : 0.0;
+  };
+
+  std::vector<double> rel(k, 0.0);
+  if (h.prm.solverType == 0) {
+    // one V-cycle: x = M b (the initial guess is not used, as in the reference)
+    precondition(h, h.kb.get(), h.kx.get(), k);
+    iters = 1;
+    true_residual(rel);
+    for (int c = 0; c < k; ++c)
+      if (rel[c] > tol) status = FEM_ERR_NOT_CONVERGED;
+  } else {
+    // r = b - A x0
+    spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kr.get(), h.kb.get(),
+                                    (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
+    dot(h, k, h.kr.get(), h.kr.get(), 0, S_RR);
+    get_scal(h, S_RR, k, rr.data());
+    bool any = false;
+    for (int c = 0; c < k; ++c) {
+      act[c] = (bb[c] > 0 && rr[c] > tol2 * bb[c]) ? 1.0 : 0.0;
+      any = any || act[c] != 0.0;
+    }
+    set_scal(h, S_ACT, k, act.data());
+    int restarts = 0;
+    while (any) {
+      precondition(h, h.kr.get(), h.kz.get(), k);
+      copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, h.kz.get(), h.kp.get());
+      HIP_LAUNCH_CHECK();
+      dot(h, k, h.kr.get(), h.kz.get(), 0, S_RZ);
+      while (iters < h.prm.maxIters) {
+        spmv_dot(h, k);     // q = A p, alpha
+        update(h, k, tol2); // x, r, rr, act
+        ++iters;
+        get_scal(h, S_ACT, k, act.data());
+        bool still = false;
+        for (int c = 0; c < k; ++c) still = still || act[c] != 0.0;
+        if (!still) break;
+        precondition(h, h.kr.get(), h.kz.get(), k);
+        dot(h, k, h.kr.get(), h.kz.get(), 1, 0);  // beta, rz
+        pcg_p_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, h.kz.get(), h.kp.get(), h.scal.get());
+        HIP_LAUNCH_CHECK();
+      }
+      // verify on the true residual; restart columns that drifted
+      true_residual(rel);
+      any = false;
+      for (int c = 0; c < k; ++c) {
+        act[c] = (rel[c] > tol) ? 1.0 : 0.0;
+        any = any || act[c] != 0.0;
+      }
+      if (!any || iters >= h.prm.maxIters || restarts >= 3) break;
+      ++restarts;
+      copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, h.kq.get(), h.kr.get());
+      HIP_LAUNCH_CHECK();
+      set_scal(h, S_ACT, k, act.data());
+    }
+    if (any) status = FEM_ERR_NOT_CONVERGED;
+    if (rel.empty() || iters == 0) true_residual(rel);
+    for (int c = 0; c < k; ++c)
+      if (rel[c] > tol) status = FEM_ERR_NOT_CONVERGED;
+  }
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (info) {
+    info->iterations = iters;
+    double mx = 0.0;
+    for (int c = 0; c < k; ++c) mx = std::max(mx, rel[c]);
+    info->rel_residual = mx;
+    info->converged = (status == FEM_OK) ? 1 : 0;
+    info->solve_ms = ms;
+  }
+  return status;
+}
+
+}  // namespace fem

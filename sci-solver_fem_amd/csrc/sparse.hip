@@ -1,0 +1,345 @@
+// Row-wise sparse products for the AMG setup (Gustavson).
+//
+// Replaces the CUSP expand-sort-reduce products of the reference
+// (cusp::multiply at src/core/cuda/smoothedMG_amg_level.cu:394,397 and
+// cusp::transpose at :382).
+//
+// Symbolic: one thread per output row inserts the row's column indices into
+// a private open-addressing hash table in LDS (256 slots).  A row that does
+// not fit is finished by a second kernel with a private table in global
+// memory sized from its product count.  Columns are then sorted per row.
+// Numeric: one thread per output row walks its products in a fixed order and
+// accumulates into the row (LDS when it fits, otherwise its global slice),
+// finding each column by binary search.  No float atomics: results are
+// bit-reproducible.
+#include "fem_internal.hpp"
+
+#include <algorithm>
+
+namespace fem {
+
+namespace {
+
+constexpr int kSymBlock = 64;
+constexpr int kSymCap = 256;             // LDS hash slots per thread (power of two)
+constexpr int kSymLimit = kSymCap * 3 / 4;
+constexpr int kNumBlock = 64;
+constexpr int kNumCap = 64;              // LDS accumulator slots per thread
+
+__device__ __forceinline__ unsigned hash_slot(int key, unsigned mask) {
+  return ((unsigned)key * 2654435761u) & mask;
+}
+
+// returns false when the table already holds `limit` keys and key is new
+__device__ __forceinline__ bool hash_insert(int* tab, unsigned mask, int key, int& count, int limit) {
+  unsigned p = hash_slot(key, mask);
+  while (true) {
+    int t = tab[p];
+    if (t == key) return true;
+    if (t == -1) {
+      if (count >= limit) return false;
+      tab[p] = key;
+      ++count;
+      return true;
+    }
+    p = (p + 1) & mask;
+  }
+}
+
+// Enumerate the output columns of row i and insert them; false on overflow.
+__device__ __forceinline__ bool row_columns(int64_t i, const int32_t* __restrict__ l_rp, const int32_t* __restrict__ l_ci,
+                                            const int32_t* __restrict__ r_rp, const int32_t* __restrict__ r_ci,
+                                            const int32_t* __restrict__ colmap, bool drop_diag, int* tab,
+                                            unsigned mask, int& count, int limit) {
+  for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
+    int k = l_ci[p];
+    if (r_rp) {
+      for (int q = r_rp[k]; q < r_rp[k + 1]; ++q) {
+        int c = colmap ? colmap[r_ci[q]] : r_ci[q];
+        if (drop_diag && c == (int)i) continue;
+        if (!hash_insert(tab, mask, c, count, limit)) return false;
+      }
+    } else {
+      int c = colmap ? colmap[k] : k;
+      if (drop_diag && c == (int)i) continue;
+      if (!hash_insert(tab, mask, c, count, limit)) return false;
+    }
+  }
+  return true;
+}
+
+// move keys to the front of tab and sort them ascending (insertion sort)
+__device__ __forceinline__ void compact_sort_small(int* tab, int cap, int count) {
+  int m = 0;
+  for (int t = 0; t < cap; ++t) {
+    int v = tab[t];
+    if (v != -1) tab[m++] = v;
+  }
+  for (int a = 1; a < count; ++a) {
+    int v = tab[a];
+    int b = a - 1;
+    while (b >= 0 && tab[b] > v) { tab[b + 1] = tab[b]; --b; }
+    tab[b + 1] = v;
+  }
+}
+
+__device__ __forceinline__ void sift_down(int* a, int start, int end) {
+  int root = start;
+  while (2 * root + 1 < end) {
+    int child = 2 * root + 1;
+    if (child + 1 < end && a[child] < a[child + 1]) ++child;
+    if (a[root] < a[child]) {
+      int t = a[root]; a[root] = a[child]; a[child] = t;
+      root = child;
+    } else {
+      return;
+    }
+  }
+}
+
+__device__ __forceinline__ void compact_sort_large(int* tab, int64_t cap, int count) {
+  int m = 0;
+  for (int64_t t = 0; t < cap; ++t) {
+    int v = tab[t];
+    if (v != -1) tab[m++] = v;
+  }
+  for (int st = count / 2 - 1; st >= 0; --st) sift_down(tab, st, count);
+  for (int end = count - 1; end > 0; --end) {
+    int t = tab[0]; tab[0] = tab[end]; tab[end] = t;
+    sift_down(tab, 0, end);
+  }
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(kSymBlock) sym_lds_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                          const int32_t* __restrict__ l_ci,
+                                                          const int32_t* __restrict__ r_rp,
+                                                          const int32_t* __restrict__ r_ci,
+                                                          const int32_t* __restrict__ colmap, int drop_diag,
+                                                          int32_t* __restrict__ cnt, const int32_t* __restrict__ c_rp,
+                                                          int32_t* __restrict__ c_ci) {
+  __shared__ int lds[kSymBlock * kSymCap];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  if (kFill && cnt[i] < 0) return;
+  int* tab = lds + threadIdx.x * kSymCap;
+  for (int t = 0; t < kSymCap; ++t) tab[t] = -1;
+  int count = 0;
+  bool ok = row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, kSymCap - 1, count, kSymLimit);
+  if (!kFill) {
+    cnt[i] = ok ? count : -1;
+    return;
+  }
+  compact_sort_small(tab, kSymCap, count);
+  int o = c_rp[i];
+  for (int t = 0; t < count; ++t) c_ci[o + t] = tab[t];
+}
+
+__global__ void collect_overflow_kernel(int64_t nrows, const int32_t* __restrict__ cnt, const int32_t* __restrict__ l_rp,
+                                        const int32_t* __restrict__ l_ci, const int32_t* __restrict__ r_rp,
+                                        int64_t ncols, int32_t* __restrict__ rows, int64_t* __restrict__ cap,
+                                        unsigned* __restrict__ nover) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows || cnt[i] >= 0) return;
+  int64_t ub = 0;
+  for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) ub += r_rp ? (r_rp[l_ci[p] + 1] - r_rp[l_ci[p]]) : 1;
+  if (ub > ncols) ub = ncols;
+  int64_t c = 1;
+  while (c < 2 * ub + 2) c <<= 1;
+  unsigned slot = atomicAdd(nover, 1u);
+  rows[slot] = (int32_t)i;
+  cap[slot] = c;
+}
+
+template <bool kFill>
+__global__ void sym_global_kernel(int64_t nover, const int32_t* __restrict__ rows, const int64_t* __restrict__ off,
+                                  const int32_t* __restrict__ l_rp, const int32_t* __restrict__ l_ci,
+                                  const int32_t* __restrict__ r_rp, const int32_t* __restrict__ r_ci,
+                                  const int32_t* __restrict__ colmap, int drop_diag, int* __restrict__ scratch,
+                                  int32_t* __restrict__ cnt, const int32_t* __restrict__ c_rp,
+                                  int32_t* __restrict__ c_ci) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nover) return;
+  int64_t i = rows[t];
+  int* tab = scratch + off[t];
+  int64_t cap = off[t + 1] - off[t];
+  for (int64_t q = 0; q < cap; ++q) tab[q] = -1;
+  int count = 0;
+  row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, (unsigned)(cap - 1), count, 0x7fffffff);
+  if (!kFill) {
+    cnt[i] = count;
+    return;
+  }
+  compact_sort_large(tab, cap, count);
+  int o = c_rp[i];
+  for (int q = 0; q < count; ++q) c_ci[o + q] = tab[q];
+}
+
+__device__ __forceinline__ int bsearch_col(const int32_t* cols, int len, int c) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cols[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kNumBlock) num_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                      const int32_t* __restrict__ l_ci, const double* __restrict__ l_v,
+                                                      const int32_t* __restrict__ r_rp, const int32_t* __restrict__ r_ci,
+                                                      const double* __restrict__ r_v, const int32_t* __restrict__ c_rp,
+                                                      const int32_t* __restrict__ c_ci, double* __restrict__ c_v) {
+  __shared__ int s_cols[kNumBlock * kNumCap];
+  __shared__ double s_acc[kNumBlock * kNumCap];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  int o = c_rp[i];
+  int len = c_rp[i + 1] - o;
+  const int32_t* cols;
+  double* acc;
+  if (len <= kNumCap) {
+    int* sc = s_cols + threadIdx.x * kNumCap;
+    acc = s_acc + threadIdx.x * kNumCap;
+    for (int t = 0; t < len; ++t) { sc[t] = c_ci[o + t]; acc[t] = 0.0; }
+    cols = sc;
+  } else {
+    cols = c_ci + o;
+    acc = c_v + o;
+    for (int t = 0; t < len; ++t) acc[t] = 0.0;
+  }
+  for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
+    int k = l_ci[p];
+    double a = l_v[p];
+    for (int q = r_rp[k]; q < r_rp[k + 1]; ++q) acc[bsearch_col(cols, len, r_ci[q])] += a * r_v[q];
+  }
+  if (len <= kNumCap)
+    for (int t = 0; t < len; ++t) c_v[o + t] = acc[t];
+}
+
+__global__ void transpose_keys_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                      uint64_t* __restrict__ key, int32_t* __restrict__ idx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = rp[i]; p < rp[i + 1]; ++p) {
+    key[p] = ((uint64_t)(uint32_t)ci[p] << 32) | (uint64_t)(uint32_t)i;
+    idx[p] = p;
+  }
+}
+
+__global__ void transpose_fill_kernel(int64_t nnz, const uint64_t* __restrict__ key, const int32_t* __restrict__ idx,
+                                      const double* __restrict__ v, int32_t* __restrict__ ci, double* __restrict__ rv) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  ci[t] = (int32_t)(key[t] & 0xffffffffull);
+  rv[t] = v[idx[t]];
+}
+
+__global__ void transpose_ptr_kernel(int64_t m, const uint64_t* __restrict__ key, int64_t nnz, int32_t* __restrict__ rp) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > m) return;
+  int64_t lo = 0, hi = nnz;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)(key[mid] >> 32) < c) lo = mid + 1; else hi = mid;
+  }
+  rp[c] = (int32_t)lo;
+}
+
+int bits_for(int64_t v) {
+  int b = 1;
+  while (b < 63 && ((int64_t)1 << b) <= v) ++b;
+  return b;
+}
+
+}  // namespace
+
+void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int64_t r_rows, const int32_t* colmap,
+                     int64_t ncols, bool drop_diag, Csr& C, hipStream_t s) {
+  (void)r_rows;
+  const int64_t n = L.n;
+  C.n = n;
+  C.m = ncols;
+  C.rp.resize(n + 1);
+  DevBuf<int32_t> cnt(n > 0 ? n : 1);
+  if (n > 0) {
+    sym_lds_kernel<false><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
+                                                                      drop_diag ? 1 : 0, cnt.get(), nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+  }
+  DevBuf<int32_t> rows(n > 0 ? n : 1);
+  DevBuf<int64_t> cap(n + 1);
+  DevBuf<unsigned> nover_d(1);
+  HIP_CHECK(hipMemsetAsync(nover_d.get(), 0, sizeof(unsigned), s));
+  if (n > 0) {
+    collect_overflow_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, cnt.get(), L.rp.get(), L.ci.get(), r_rp, ncols,
+                                                             rows.get(), cap.get(), nover_d.get());
+    HIP_LAUNCH_CHECK();
+  }
+  unsigned nover = 0;
+  HIP_CHECK(hipMemcpyAsync(&nover, nover_d.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  DevBuf<int64_t> off;
+  DevBuf<int> scratch;
+  if (nover) {
+    off.resize(nover + 1);
+    int64_t tot = dev_exclusive_scan_i64(cap.get(), off.get(), nover, s);
+    scratch.resize(tot);
+    sym_global_kernel<false><<<grid_for(nover, 64), 64, 0, s>>>(nover, rows.get(), off.get(), L.rp.get(), L.ci.get(),
+                                                                r_rp, r_ci, colmap, drop_diag ? 1 : 0, scratch.get(),
+                                                                cnt.get(), nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+  }
+  int64_t nnz = dev_exclusive_scan_i32(cnt.get(), C.rp.get(), n, s);
+  if (nnz >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "sparse product has 2^31 or more nonzeros");
+  C.nnz = nnz;
+  C.ci.resize(nnz > 0 ? nnz : 1);
+  if (n > 0) {
+    sym_lds_kernel<true><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
+                                                                     drop_diag ? 1 : 0, cnt.get(), C.rp.get(),
+                                                                     C.ci.get());
+    HIP_LAUNCH_CHECK();
+  }
+  if (nover) {
+    sym_global_kernel<true><<<grid_for(nover, 64), 64, 0, s>>>(nover, rows.get(), off.get(), L.rp.get(), L.ci.get(),
+                                                               r_rp, r_ci, colmap, drop_diag ? 1 : 0, scratch.get(),
+                                                               cnt.get(), C.rp.get(), C.ci.get());
+    HIP_LAUNCH_CHECK();
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
+  C.v64.resize(C.nnz > 0 ? C.nnz : 1);
+  if (C.n == 0) return;
+  num_kernel<<<grid_for(C.n, kNumBlock), kNumBlock, 0, s>>>(C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(),
+                                                            Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(),
+                                                            C.v64.get());
+  HIP_LAUNCH_CHECK();
+}
+
+void csr_transpose(const Csr& P, Csr& R, hipStream_t s) {
+  const int64_t nnz = P.nnz;
+  R.n = P.m;
+  R.m = P.n;
+  R.nnz = nnz;
+  R.rp.resize(R.n + 1);
+  R.ci.resize(nnz > 0 ? nnz : 1);
+  R.v64.resize(nnz > 0 ? nnz : 1);
+  DevBuf<uint64_t> key(nnz > 0 ? nnz : 1), key2(nnz > 0 ? nnz : 1);
+  DevBuf<int32_t> idx(nnz > 0 ? nnz : 1), idx2(nnz > 0 ? nnz : 1);
+  if (P.n > 0) {
+    transpose_keys_kernel<<<grid_for(P.n, 256), 256, 0, s>>>(P.n, P.rp.get(), P.ci.get(), key.get(), idx.get());
+    HIP_LAUNCH_CHECK();
+  }
+  dev_sort_pairs_u64_i32(key.get(), key2.get(), idx.get(), idx2.get(), (size_t)nnz, 32 + bits_for(P.m), s);
+  if (nnz > 0) {
+    transpose_fill_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, key2.get(), idx2.get(), P.v64.get(), R.ci.get(),
+                                                             R.v64.get());
+    HIP_LAUNCH_CHECK();
+  }
+  transpose_ptr_kernel<<<grid_for(R.n + 1, 256), 256, 0, s>>>(R.n, key2.get(), nnz, R.rp.get());
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace fem

@@ -1,0 +1,172 @@
+"""GPU AMG-PCG parity: solutions against an FP64 direct solve (scipy SuperLU on
+the oracle's matrix), the true-residual stop test, mixed precision, batched
+right-hand sides, preconditioner symmetry and the reference's own
+end-to-end fixtures (src/test/sanity2D.cc, sanity3D.cc, tetVol.cc)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from conftest import REFDATA, report
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(meshgen, oracle, N, sigma, mass, seed=0):
+    xyz, conn = meshgen.cube_mesh(N, seed=seed)
+    kappa = meshgen.lognormal_kappa(seed, conn.shape[0], sigma)
+    A = oracle.scipy_matrix(xyz, conn, kappa, mass)
+    b = meshgen.gaussian_rhs(seed, xyz.shape[0])
+    return xyz, conn, kappa, A, b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("smoother", [0, 1])
+def test_pcg_matches_direct_solve(fem, gpu, oracle, meshgen, precision, smoother):
+    report(f"[gpu] pcg N=13 precision={precision} smoother={smoother}")
+    xyz, conn, kappa, A, b = _problem(meshgen, oracle, 13, 1.0, 1e-2, seed=1)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-10, maxIters=300, precision=precision, smootherType=smoother,
+                 preInnerIters=2, postInnerIters=2, topSize=64)
+    x, info = s.solve(b)
+    xd = spla.spsolve(A.tocsc(), b)
+    rel_res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    report(f"      iters={info['iterations']} levels={info['num_levels']} res={rel_res:.2e}")
+    assert info["status"] == 0 and info["converged"] == 1
+    assert rel_res <= 1e-10
+    assert abs(info["rel_residual"] - rel_res) <= 1e-3 * rel_res + 1e-15
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-7
+    assert info["num_levels"] >= 2
+    s.close()
+
+
+def test_batched_rhs_equal_single(fem, gpu, oracle, meshgen):
+    report("[gpu] batched k=4 vs single")
+    xyz, conn, kappa, A, _ = _problem(meshgen, oracle, 11, 1.5, 1e-2, seed=2)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-9, maxIters=300, precision=1, preInnerIters=2, postInnerIters=2,
+                 topSize=64)
+    B = meshgen.gaussian_rhs(5, xyz.shape[0], 4)
+    X, info = s.solve(B)
+    assert info["status"] == 0
+    for c in range(4):
+        x1, _ = s.solve(np.ascontiguousarray(B[:, c]))
+        r = np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c])
+        assert r <= 1e-9
+        assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) <= 1e-7
+    s.close()
+
+
+@pytest.mark.parametrize("smoother", [0, 1])
+def test_preconditioner_symmetric_positive(fem, gpu, meshgen, smoother):
+    report(f"[gpu] V-cycle symmetry smoother={smoother}")
+    xyz, conn = meshgen.cube_mesh(10, seed=3)
+    kappa = meshgen.lognormal_kappa(3, conn.shape[0], 1.0)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(precision=0, smootherType=smoother, preInnerIters=2, postInnerIters=2, topSize=64)
+    rng = np.random.default_rng(7)
+    u = rng.normal(size=xyz.shape[0])
+    v = rng.normal(size=xyz.shape[0])
+    Mu, Mv = s.apply_preconditioner(u), s.apply_preconditioner(v)
+    assert abs(Mu @ v - u @ Mv) <= 1e-10 * abs(Mu @ v)
+    assert Mu @ u > 0 and Mv @ v > 0
+    s.close()
+
+
+def test_one_vcycle_mode(fem, gpu, oracle, meshgen):
+    """solverType 0 applies exactly one V-cycle (src/core/cuda/amg.cu:193)."""
+    report("[gpu] solverType 0")
+    xyz, conn, kappa, A, b = _problem(meshgen, oracle, 9, 0.5, 1.0, seed=4)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1.0)
+    s.set_params(solverType=0, tolerance=1e-6, topSize=32)
+    x, info = s.solve(b)
+    assert info["iterations"] == 1
+    r = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert r < 0.5 and abs(info["rel_residual"] - r) < 1e-9
+    assert np.allclose(x, s.apply_preconditioner(b), rtol=1e-12, atol=1e-14)
+    s.close()
+
+
+def test_square_c1_like(fem, gpu, oracle, meshgen):
+    report("[gpu] 2D sine load N=33")
+    xyz, tris = meshgen.square_mesh(33, seed=0)
+    b = meshgen.sine_load_2d(xyz, tris)
+    A = oracle.scipy_matrix(xyz, tris, None, 1.0)
+    s = fem.FEMSolver.from_arrays(2, xyz, tris, mass_coeff=1.0)
+    s.set_params(solverType=1, tolerance=1e-10, maxIters=200, topSize=64)
+    x, info = s.solve(b)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert info["status"] == 0
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-8
+    s.close()
+
+
+def test_not_converged_status(fem, gpu, meshgen):
+    report("[gpu] maxIters exhausted")
+    xyz, conn = meshgen.cube_mesh(9, seed=5)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-14, maxIters=2, topSize=64)
+    b = meshgen.gaussian_rhs(1, xyz.shape[0])
+    x, info = s.solve(b)
+    assert info["status"] == fem._lib.FEM_ERR_NOT_CONVERGED and info["converged"] == 0
+    assert info["iterations"] == 2 and np.all(np.isfinite(x))
+    with pytest.raises(fem.NotConverged):
+        s.solve(b, raise_on_nonconvergence=True)
+    s.close()
+
+
+# ---- the reference's end-to-end tests (same tolerances), plus tight versions ----
+
+def test_reference_sanity3d(fem, gpu):
+    """src/test/sanity3D.cc: simple mesh + simple.mat (79.9568 I), ||x - ans|| < 1."""
+    report("[gpu] sanity3D")
+    cfg = fem.FEMSolver(str(REFDATA / "simple"), True, False)
+    assert cfg.readMatlabSparseMatrix(str(REFDATA / "simple.mat")) == 0
+    b = fem.read_matlab_array(str(REFDATA / "simpleb.mat"))
+    x = np.zeros(cfg.getMatrixRows())
+    cfg.solveFEM(x, b)
+    ans = fem.read_matlab_array(str(REFDATA / "simpleAns.mat"))
+    assert np.linalg.norm(x - ans) < 1.0
+    # tight: PCG to 1e-10 reproduces the fixture answer (itself A^-1 b to 2.3e-8)
+    cfg.solverType_ = 1
+    cfg.tolerance_ = 1e-10
+    x2 = np.zeros_like(x)
+    cfg.solveFEM(x2, b)
+    assert np.linalg.norm(x2 - ans) / np.linalg.norm(ans) < 1e-6
+    cfg.close()
+
+
+def test_reference_tetvol(fem, gpu):
+    """src/test/tetVol.cc: SCIRun matrix on the tetVol mesh, ||x - ans|| < 25."""
+    report("[gpu] tetVol")
+    cfg = fem.FEMSolver(str(REFDATA / "tetVol"), True, False)
+    assert cfg.readMatlabSparseMatrix(str(REFDATA / "tetVolA.mat")) == 0
+    b = fem.read_matlab_array(str(REFDATA / "tetVolb.mat"))
+    x = np.zeros(cfg.getMatrixRows())
+    cfg.solveFEM(x, b)
+    ans = fem.read_matlab_array(str(REFDATA / "tetVolAns.mat"))
+    assert np.linalg.norm(x - ans) < 25.0
+    cfg.solverType_ = 1
+    cfg.tolerance_ = 1e-10
+    cfg.maxIters_ = 500
+    x2 = np.zeros_like(x)
+    info = cfg.solveFEM(x2, b)
+    report(f"      tetVol PCG iters={info['iterations']} res={info['rel_residual']:.2e}")
+    assert info["status"] == 0
+    # the fixture answer itself has relative residual 6.7e-6
+    assert np.linalg.norm(x2 - ans) / np.linalg.norm(ans) < 1e-4
+    cfg.close()
+
+
+def test_reference_sanity2d(fem, gpu):
+    """src/test/sanity2D.cc: simple.ply + simpleTri.mat, ||x - ans|| < 100."""
+    report("[gpu] sanity2D")
+    cfg = fem.FEMSolver(str(REFDATA / "simple.ply"), False, False)
+    assert cfg.readMatlabSparseMatrix(str(REFDATA / "simpleTri.mat")) == 0
+    b = fem.read_matlab_array(str(REFDATA / "simpleTrib.mat"))
+    x = np.zeros(cfg.getMatrixRows())
+    cfg.solveFEM(x, b)
+    ans = fem.read_matlab_array(str(REFDATA / "simpleTriAns.mat"))
+    assert np.linalg.norm(x - ans) < 100.0
+    cfg.close()
