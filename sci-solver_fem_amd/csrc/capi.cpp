@@ -52,8 +52,6 @@ void init_device(Handle& h) {
   HIP_CHECK(hipSetDevice(h.device));
   HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
   h.own_stream = true;
-  HIP_CHECK(hipEventCreate(&h.ev0));
-  HIP_CHECK(hipEventCreate(&h.ev1));
 }
 
 // material label -> element coefficient (perform_element_loop_3D.cuh:585-610)
@@ -277,8 +275,7 @@ void fem_destroy(fem_handle* hh) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->h_scal) (void)hipHostFree(h->h_scal);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  release_profile_events(*h);
   hipStream_t s = h->own_stream ? h->stream : nullptr;
   delete h;  // device buffers free themselves
   if (s) (void)hipStreamDestroy(s);
@@ -616,16 +613,24 @@ int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj, 
 
 int fem_profile_enable(fem_handle* hh, int enable) {
   if (!hh) return FEM_ERR_ARG;
-  Handle& h = *H(hh);
-  h.profiling = enable != 0;
-  if (enable) h.stats.clear();
-  return FEM_OK;
+  return guarded(hh, [&]() {
+    Handle& h = *H(hh);
+    flush_profile(h);
+    h.profiling = enable != 0;
+    if (enable) h.stats.clear();
+    return (int)FEM_OK;
+  });
 }
 
 int fem_profile_get(const fem_handle* hc, const char* family, int64_t* launches, double* total_ms,
                     double* bytes_per_launch) {
   if (!hc || !family) return FEM_ERR_ARG;
-  const Handle& h = *H(hc);
+  Handle& h = *H(const_cast<fem_handle*>(hc));
+  int st = guarded(const_cast<fem_handle*>(hc), [&]() {
+    flush_profile(h);
+    return (int)FEM_OK;
+  });
+  if (st != FEM_OK) return st;
   auto it = h.stats.find(family);
   if (it == h.stats.end()) {
     if (launches) *launches = 0;

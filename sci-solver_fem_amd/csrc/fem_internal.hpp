@@ -65,6 +65,12 @@ struct KernelStat {
   double bytes = 0;
 };
 
+struct PendingTiming {
+  std::string fam;
+  hipEvent_t a, b;
+  double bytes;
+};
+
 struct Handle {
   std::string err;
   fem_params prm{};
@@ -104,14 +110,14 @@ struct Handle {
   int kcap = 0;
   DevBuf<double> kx, kr, kp, kq, kz, kb;
   DevBuf<double> partials;        // reduction partials
-  DevBuf<unsigned> counters;      // last-block counters
   DevBuf<double> scal;            // device scalars
   double* h_scal = nullptr;       // pinned mirror of scal
   DevBuf<double> ubuf, ubuf2;     // user-order staging (nv * k)
 
   bool profiling = false;
   std::map<std::string, KernelStat> stats;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<PendingTiming> pending;   // recorded, not yet read back
+  std::vector<hipEvent_t> ev_pool;      // free events
 };
 
 // ---- primitives (primitives.hip) ----
@@ -157,14 +163,19 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info);   // uses h.kb (rhs), h.k
 void apply_vcycle_fp64(Handle& h, const double* r, double* z);  // level-0 preconditioner, 1 column
 void apply_operator_int(Handle& h, const double* x, double* y, int k);
 
-// profiling helper: records kernel time into h.stats when enabled
+// Profiling: when enabled, HIP events bracket each launch of a kernel family
+// on the handle's stream; they are read back (without stalling the stream
+// between launches) by flush_profile.
 struct ProfScope {
   Handle& h;
   const char* fam;
   double bytes;
   bool on;
+  hipEvent_t a = nullptr;
   ProfScope(Handle& hh, const char* f, double b);
   ~ProfScope();
 };
+void flush_profile(Handle& h);
+void release_profile_events(Handle& h);
 
 }  // namespace fem

@@ -106,11 +106,6 @@ __device__ __forceinline__ double block_sum(double v, double* s_buf) {
   return r;
 }
 
-__device__ __forceinline__ double load_agent(const double* p) {
-  unsigned long long u = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
 // Per-column block partials for row-major (n x k) data: thread (row, c) with
 // c = tid % KP.  Writes partials[blockIdx.x * k + c] for c < k.
 template <int KP>
@@ -128,28 +123,6 @@ __device__ __forceinline__ void block_col_partials(double v, int k, double* s_bu
     partials[(int64_t)blockIdx.x * k + threadIdx.x] = s;
   }
   __syncthreads();
-}
-
-// Returns true in every thread of the block that finished last; that block
-// may then read all partials (agent-scope loads) and must reset the counter.
-__device__ __forceinline__ bool last_block(unsigned* counter, unsigned* s_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  bool last = (*s_flag != 0u);
-  if (last) __threadfence();
-  return last;
-}
-
-// total over nb partials for column c (called by the whole last block)
-__device__ __forceinline__ double finish_col(const double* partials, int nb, int k, int c, double* s_buf) {
-  double v = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kNT) v += load_agent(partials + (int64_t)b * k + c);
-  return block_sum(v, s_buf);
 }
 
 }  // namespace fem

@@ -24,7 +24,6 @@ namespace fem {
 namespace {
 
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_ACT, S_RZN, S_TRUE, S_COUNT };
-enum { C_PQ = 0, C_RR, C_DOT, C_COUNT };  // last-block counters
 
 int pow2_at_least(int k) {
   int p = 1;
@@ -154,16 +153,18 @@ __global__ void __launch_bounds__(64) coarse_kernel(int64_t nc, int k, const T* 
 }
 
 // ---- PCG vector kernels (FP64, k columns, row-major) --------------------------
+// Each reduction kernel writes one partial per block and column; a one-block
+// finish kernel sums them in a fixed order and updates the device scalars.
+// (A last-block-done pattern inside the producer would need an agent-scope
+// release fence per block, which writes back the XCD's L2 on gfx950.)
 
-// q = A p and per-column p.q; last block: alpha = act ? rz / pq : 0
+// q = A p and per-block partials of p.q (one column)
 template <int G>
 __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int32_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci, const double* __restrict__ val,
                                                          const double* __restrict__ p, double* __restrict__ q,
-                                                         double* __restrict__ partials, unsigned* __restrict__ counter,
-                                                         double* __restrict__ scal) {
+                                                         double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
-  __shared__ unsigned s_flag;
   double part = 0.0;
   const int lane = threadIdx.x & (G - 1);
   const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / G);
@@ -180,24 +181,15 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
   }
   double tot = block_sum(part, s_buf);
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
-  if (last_block(counter, &s_flag)) {
-    double pq = finish_col(partials, gridDim.x, 1, 0, s_buf);
-    if (threadIdx.x == 0) {
-      scal[S_PQ] = pq;
-      scal[S_ALPHA] = (scal[S_ACT] != 0.0 && pq != 0.0) ? scal[S_RZ] / pq : 0.0;
-      *counter = 0u;
-    }
-  }
 }
 
+// q = A p and per-block, per-column partials of p.q (k columns)
 template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci,
                                                          const double* __restrict__ val, const double* __restrict__ p,
-                                                         double* __restrict__ q, double* __restrict__ partials,
-                                                         unsigned* __restrict__ counter, double* __restrict__ scal) {
+                                                         double* __restrict__ q, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
-  __shared__ unsigned s_flag;
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
   const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
@@ -211,27 +203,15 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
     }
   }
   block_col_partials<KP>(part, k, s_buf, partials);
-  if (last_block(counter, &s_flag)) {
-    for (int cc = 0; cc < k; ++cc) {
-      double pq = finish_col(partials, gridDim.x, k, cc, s_buf);
-      if (threadIdx.x == 0) {
-        scal[S_PQ * k + cc] = pq;
-        scal[S_ALPHA * k + cc] = (scal[S_ACT * k + cc] != 0.0 && pq != 0.0) ? scal[S_RZ * k + cc] / pq : 0.0;
-      }
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-  }
 }
 
-// x += alpha p; r -= alpha q; rr = r.r; last block: act &= rr > tol2 * bb
+// x += alpha p; r -= alpha q; partials of r.r
 template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, double* __restrict__ x,
                                                        double* __restrict__ r, const double* __restrict__ p,
                                                        const double* __restrict__ q, double* __restrict__ partials,
-                                                       unsigned* __restrict__ counter, double* __restrict__ scal,
-                                                       double tol2) {
+                                                       const double* __restrict__ scal) {
   __shared__ double s_buf[kNT];
-  __shared__ unsigned s_flag;
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
   const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
@@ -246,27 +226,13 @@ __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, doubl
     }
   }
   block_col_partials<KP>(part, k, s_buf, partials);
-  if (last_block(counter, &s_flag)) {
-    for (int cc = 0; cc < k; ++cc) {
-      double rr = finish_col(partials, gridDim.x, k, cc, s_buf);
-      if (threadIdx.x == 0) {
-        scal[S_RR * k + cc] = rr;
-        bool act = scal[S_ACT * k + cc] != 0.0 && rr > tol2 * scal[S_BB * k + cc];
-        scal[S_ACT * k + cc] = act ? 1.0 : 0.0;
-      }
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-  }
 }
 
-// per-column a.b; finish: 0 -> scal[slot] = dot; 1 -> beta = act ? dot/rz : 0, rz = dot
-template <int KP, int FIN>
+// partials of a.b per column
+template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const double* __restrict__ a,
-                                                    const double* __restrict__ b, double* __restrict__ partials,
-                                                    unsigned* __restrict__ counter, double* __restrict__ scal,
-                                                    int slot) {
+                                                    const double* __restrict__ b, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
-  __shared__ unsigned s_flag;
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
   const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
@@ -274,20 +240,34 @@ __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const do
     for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid)
       part += a[row * k + c] * b[row * k + c];
   block_col_partials<KP>(part, k, s_buf, partials);
-  if (last_block(counter, &s_flag)) {
-    for (int cc = 0; cc < k; ++cc) {
-      double v = finish_col(partials, gridDim.x, k, cc, s_buf);
-      if (threadIdx.x == 0) {
-        if (FIN == 0) {
-          scal[slot * k + cc] = v;
-        } else {
-          double rz = scal[S_RZ * k + cc];
-          scal[S_BETA * k + cc] = (scal[S_ACT * k + cc] != 0.0 && rz != 0.0) ? v / rz : 0.0;
-          scal[S_RZ * k + cc] = v;
-        }
+}
+
+enum FinishMode { F_STORE = 0, F_ALPHA = 1, F_RR = 2, F_BETA = 3 };
+
+// one block: column totals of nb partials, then the scalar update
+__global__ void __launch_bounds__(kNT) pcg_finish_kernel(int nb, int k, const double* __restrict__ partials,
+                                                       double* __restrict__ scal, int mode, int slot, double tol2) {
+  __shared__ double s_buf[kNT];
+  for (int c = 0; c < k; ++c) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kNT) v += partials[(int64_t)b * k + c];
+    v = block_sum(v, s_buf);
+    if (threadIdx.x == 0) {
+      if (mode == F_STORE) {
+        scal[slot * k + c] = v;
+      } else if (mode == F_ALPHA) {
+        scal[S_PQ * k + c] = v;
+        scal[S_ALPHA * k + c] = (scal[S_ACT * k + c] != 0.0 && v != 0.0) ? scal[S_RZ * k + c] / v : 0.0;
+      } else if (mode == F_RR) {
+        scal[S_RR * k + c] = v;
+        bool act = scal[S_ACT * k + c] != 0.0 && v > tol2 * scal[S_BB * k + c];
+        scal[S_ACT * k + c] = act ? 1.0 : 0.0;
+      } else {
+        double rz = scal[S_RZ * k + c];
+        scal[S_BETA * k + c] = (scal[S_ACT * k + c] != 0.0 && rz != 0.0) ? v / rz : 0.0;
+        scal[S_RZ * k + c] = v;
       }
     }
-    if (threadIdx.x == 0) *counter = 0u;
   }
 }
 
@@ -444,19 +424,58 @@ void precondition(Handle& h, const double* r, double* z, int k) {
 
 }  // namespace
 
-ProfScope::ProfScope(Handle& hh, const char* f, double b) : h(hh), fam(f), bytes(b), on(hh.profiling && f && f[0]) {
-  if (on) HIP_CHECK(hipEventRecord(h.ev0, h.stream));
+namespace {
+hipEvent_t take_event(Handle& h) {
+  if (!h.ev_pool.empty()) {
+    hipEvent_t e = h.ev_pool.back();
+    h.ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
 }
+}  // namespace
+
+ProfScope::ProfScope(Handle& hh, const char* f, double b) : h(hh), fam(f), bytes(b), on(hh.profiling && f && f[0]) {
+  if (!on) return;
+  a = take_event(h);
+  HIP_CHECK(hipEventRecord(a, h.stream));
+}
+
 ProfScope::~ProfScope() {
   if (!on) return;
-  if (hipEventRecord(h.ev1, h.stream) != hipSuccess) return;
-  if (hipEventSynchronize(h.ev1) != hipSuccess) return;
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, h.ev0, h.ev1) != hipSuccess) return;
-  KernelStat& st = h.stats[fam];
-  st.launches += 1;
-  st.total_ms += ms;
-  st.bytes = bytes;
+  hipEvent_t b = take_event(h);
+  if (hipEventRecord(b, h.stream) != hipSuccess) return;
+  h.pending.push_back(PendingTiming{fam, a, b, bytes});
+  if (h.pending.size() >= 8192) flush_profile(h);
+}
+
+void flush_profile(Handle& h) {
+  if (h.pending.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(h.stream));
+  for (auto& p : h.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      KernelStat& st = h.stats[p.fam];
+      st.launches += 1;
+      st.total_ms += ms;
+      st.bytes = p.bytes;
+    }
+    h.ev_pool.push_back(p.a);
+    h.ev_pool.push_back(p.b);
+  }
+  h.pending.clear();
+}
+
+void release_profile_events(Handle& h) {
+  for (auto& p : h.pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  h.pending.clear();
+  for (auto e : h.ev_pool) (void)hipEventDestroy(e);
+  h.ev_pool.clear();
 }
 
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s) {
@@ -476,8 +495,6 @@ void ensure_krylov(Handle& h, int k) {
     h.kz.resize(n * k); h.kb.resize(n * k);
     h.ubuf.resize(n * k); h.ubuf2.resize(n * k);
     h.partials.resize((size_t)4096 * 64);
-    h.counters.resize(C_COUNT);
-    HIP_CHECK(hipMemsetAsync(h.counters.get(), 0, sizeof(unsigned) * C_COUNT, h.stream));
     h.scal.resize((size_t)S_COUNT * 64);
     if (!h.h_scal) HIP_CHECK(hipHostMalloc((void**)&h.h_scal, sizeof(double) * S_COUNT * 64, hipHostMallocDefault));
     h.kcap = k;
@@ -499,38 +516,41 @@ void apply_vcycle_fp64(Handle& h, const double* r, double* z) { precondition(h, 
 
 namespace {
 
-template <int KP>
-void launch_dot(Handle& h, int k, const double* a, const double* b, int fin, int slot) {
-  const int64_t n = h.A.n;
-  unsigned g = vec_grid(n, KP);
-  if (fin == 0)
-    pcg_dot_kernel<KP, 0><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get(), h.counters.get() + C_DOT,
-                                                   h.scal.get(), slot);
-  else
-    pcg_dot_kernel<KP, 1><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get(), h.counters.get() + C_DOT,
-                                                   h.scal.get(), slot);
+void finish(Handle& h, int nb, int k, int mode, int slot, double tol2 = 0.0) {
+  pcg_finish_kernel<<<1, kNT, 0, h.stream>>>(nb, k, h.partials.get(), h.scal.get(), mode, slot, tol2);
   HIP_LAUNCH_CHECK();
 }
 
-void dot(Handle& h, int k, const double* a, const double* b, int fin, int slot) {
+template <int KP>
+void launch_dot(Handle& h, int k, const double* a, const double* b, int mode, int slot) {
+  const int64_t n = h.A.n;
+  unsigned g = vec_grid(n, KP);
+  pcg_dot_kernel<KP><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get());
+  HIP_LAUNCH_CHECK();
+  finish(h, (int)g, k, mode, slot);
+}
+
+// mode F_STORE: scal[slot] = a.b; F_BETA: beta = a.b / rz, rz = a.b
+void dot(Handle& h, int k, const double* a, const double* b, int mode, int slot) {
   switch (pow2_at_least(k)) {
-    case 1: launch_dot<1>(h, k, a, b, fin, slot); break;
-    case 2: launch_dot<2>(h, k, a, b, fin, slot); break;
-    case 4: launch_dot<4>(h, k, a, b, fin, slot); break;
-    case 8: launch_dot<8>(h, k, a, b, fin, slot); break;
-    case 16: launch_dot<16>(h, k, a, b, fin, slot); break;
-    case 32: launch_dot<32>(h, k, a, b, fin, slot); break;
-    default: launch_dot<64>(h, k, a, b, fin, slot); break;
+    case 1: launch_dot<1>(h, k, a, b, mode, slot); break;
+    case 2: launch_dot<2>(h, k, a, b, mode, slot); break;
+    case 4: launch_dot<4>(h, k, a, b, mode, slot); break;
+    case 8: launch_dot<8>(h, k, a, b, mode, slot); break;
+    case 16: launch_dot<16>(h, k, a, b, mode, slot); break;
+    case 32: launch_dot<32>(h, k, a, b, mode, slot); break;
+    default: launch_dot<64>(h, k, a, b, mode, slot); break;
   }
 }
 
 template <int KP>
 void launch_update(Handle& h, int k, double tol2) {
   const int64_t n = h.A.n;
-  pcg_update_kernel<KP><<<vec_grid(n, KP), kNT, 0, h.stream>>>(n, k, h.kx.get(), h.kr.get(), h.kp.get(), h.kq.get(),
-                                                               h.partials.get(), h.counters.get() + C_RR,
-                                                               h.scal.get(), tol2);
+  unsigned g = vec_grid(n, KP);
+  pcg_update_kernel<KP><<<g, kNT, 0, h.stream>>>(n, k, h.kx.get(), h.kr.get(), h.kp.get(), h.kq.get(),
+                                                 h.partials.get(), h.scal.get());
   HIP_LAUNCH_CHECK();
+  finish(h, (int)g, k, F_RR, 0, tol2);
 }
 
 void update(Handle& h, int k, double tol2) {
@@ -551,9 +571,9 @@ void launch_spmv_dot(Handle& h) {
   int64_t g = (A.n * G + kNT - 1) / kNT;
   if (g > 4096) g = 4096;
   pcg_spmv_dot_kernel<G><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, A.rp.get(), A.ci.get(), A.v64.get(), h.kp.get(),
-                                                            h.kq.get(), h.partials.get(), h.counters.get() + C_PQ,
-                                                            h.scal.get());
+                                                            h.kq.get(), h.partials.get());
   HIP_LAUNCH_CHECK();
+  finish(h, (int)g, 1, F_ALPHA, 0);
 }
 
 template <int KP>
@@ -562,9 +582,9 @@ void launch_spmm_dot(Handle& h, int k) {
   int64_t g = (A.n * KP + kNT - 1) / kNT;
   if (g > 4096) g = 4096;
   pcg_spmm_dot_kernel<KP><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, k, A.rp.get(), A.ci.get(), A.v64.get(),
-                                                             h.kp.get(), h.kq.get(), h.partials.get(),
-                                                             h.counters.get() + C_PQ, h.scal.get());
+                                                             h.kp.get(), h.kq.get(), h.partials.get());
   HIP_LAUNCH_CHECK();
+  finish(h, (int)g, k, F_ALPHA, 0);
 }
 
 void spmv_dot(Handle& h, int k) {
@@ -616,7 +636,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
   HIP_CHECK(hipEventCreate(&e1));
   HIP_CHECK(hipEventRecord(e0, s));
 
-  dot(h, k, h.kb.get(), h.kb.get(), 0, S_BB);
+  dot(h, k, h.kb.get(), h.kb.get(), F_STORE, S_BB);
   get_scal(h, S_BB, k, bb.data());
 
   int iters = 0;
@@ -625,7 +645,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     // kq = b - A x ; rel_c = ||kq_c|| / ||b_c||
     spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kq.get(), h.kb.get(), (const double*)nullptr,
                                     nullptr, nullptr, 0.0, 0.0, s);
-    dot(h, k, h.kq.get(), h.kq.get(), 0, S_TRUE);
+    dot(h, k, h.kq.get(), h.kq.get(), F_STORE, S_TRUE);
     std::vector<double> t(k);
     get_scal(h, S_TRUE, k, t.data());
     rel.resize(k);
@@ -644,7 +664,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     // r = b - A x0
     spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kr.get(), h.kb.get(),
                                     (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
-    dot(h, k, h.kr.get(), h.kr.get(), 0, S_RR);
+    dot(h, k, h.kr.get(), h.kr.get(), F_STORE, S_RR);
     get_scal(h, S_RR, k, rr.data());
     bool any = false;
     for (int c = 0; c < k; ++c) {
@@ -657,7 +677,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       precondition(h, h.kr.get(), h.kz.get(), k);
       copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, h.kz.get(), h.kp.get());
       HIP_LAUNCH_CHECK();
-      dot(h, k, h.kr.get(), h.kz.get(), 0, S_RZ);
+      dot(h, k, h.kr.get(), h.kz.get(), F_STORE, S_RZ);
       while (iters < h.prm.maxIters) {
         spmv_dot(h, k);     // q = A p, alpha
         update(h, k, tol2); // x, r, rr, act
@@ -667,7 +687,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
         for (int c = 0; c < k; ++c) still = still || act[c] != 0.0;
         if (!still) break;
         precondition(h, h.kr.get(), h.kz.get(), k);
-        dot(h, k, h.kr.get(), h.kz.get(), 1, 0);  // beta, rz
+        dot(h, k, h.kr.get(), h.kz.get(), F_BETA, 0);  // beta, rz
         pcg_p_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, h.kz.get(), h.kp.get(), h.scal.get());
         HIP_LAUNCH_CHECK();
       }
