@@ -387,3 +387,211 @@ void orc_element_grad(int dim, const double* xyz, int64_t ne, const int32_t* con
     out[e] = -s;
   }
 }
+
+/* ========================================================================= */
+/* Aggregation: randomized distance-k MIS + node allocation                  */
+/* ========================================================================= */
+
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* seed of generator thread i; replaces rand() after srand(time(NULL))
+ * (src/core/cuda/randomizedMIS_GPU.cu:195-198) with a seeded stream */
+uint32_t orc_mis_seed(uint64_t seed, uint32_t i) {
+  return (uint32_t)(mix64(seed * 0x9E3779B97F4A7C15ull + (uint64_t)(i + 1) * 0xD1B54A32D192ED03ull) >> 32);
+}
+
+static uint32_t taus_step(uint32_t z) {
+  uint32_t b = (((z << 13) ^ z) >> 19);
+  return ((z & 0xffffffffu) << 12) ^ b;
+}
+
+#define MIS_THREADS 32768
+#define MIS_IN 1000001
+
+/* misHelpers::randomizedMIS (src/core/cuda/randomizedMIS_GPU.cu:154-272) with
+ * Generate_Randoms_Kernel (:3-20), First_Initialize_Kernel (:22-34),
+ * Initialize_Kernel (:36-62), Iterate_Kernel (:64-99), Final_Iterate_Kernel
+ * (:101-152).  All hops are Jacobi updates (read old arrays, write new). */
+void orc_mis(int64_t n, const int32_t* ap, const int32_t* aj, int depth, uint64_t seed, int32_t* mis) {
+  uint32_t* rnd = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+  int32_t *bi = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  int32_t *bo = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  int32_t *oi = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  int32_t *oo = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  if (depth < 1) depth = 1;
+  for (int64_t i = 0; i < n; ++i) mis[i] = -1;
+  for (int t = 0; t < MIS_THREADS; ++t) {
+    uint32_t z = orc_mis_seed(seed, (uint32_t)t);
+    for (int64_t off = t; off < n; off += MIS_THREADS) {
+      z = taus_step(z);
+      rnd[off] = z;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    oi[i] = (int32_t)i;
+    bi[i] = (int32_t)(rnd[i] % 1000000u);
+  }
+  int first = 1;
+  for (;;) {
+    if (!first) {
+      for (int64_t i = 0; i < n; ++i) {
+        uint32_t z = taus_step(rnd[i]);
+        oi[i] = (int32_t)i;
+        bi[i] = mis[i] == -1 ? (int32_t)(z % 1000000u) : (mis[i] == 1 ? MIS_IN : 0);
+        rnd[i] = z;
+      }
+    }
+    first = 0;
+    int incomplete = 0;
+    for (int h = 0; h < depth; ++h) {
+      const int final_hop = (h == depth - 1);
+#pragma omp parallel for schedule(static) reduction(| : incomplete)
+      for (int64_t i = 0; i < n; ++i) {
+        int32_t best = bi[i], org = oi[i];
+        if (best < MIS_IN) {
+          for (int32_t q = ap[i]; q < ap[i + 1]; ++q) {
+            int32_t nb = aj[q], ch = bi[nb], cho = oi[nb];
+            if (ch > 0 && ch == best && cho > org) org = cho;
+            if (ch > best) {
+              best = ch;
+              org = cho;
+            }
+          }
+        }
+        if (!final_hop) {
+          bo[i] = best;
+          oo[i] = org;
+        } else {
+          int32_t st = -1;
+          if (org == (int32_t)i) st = 1;
+          else if (best == MIS_IN) st = 0;
+          mis[i] = st;
+          if (st == -1) incomplete = 1;
+        }
+      }
+      if (!final_hop) {
+        int32_t* t = bi; bi = bo; bo = t;
+        t = oi; oi = oo; oo = t;
+      }
+    }
+    if (!incomplete) break;
+  }
+  free(rnd); free(bi); free(bo); free(oi); free(oo);
+}
+
+/* misHelpers::aggregateGraph (src/core/cuda/misHelpers.cu:513-601) with
+ * allocateNodesKernel (:13-75) and removeRuntyParts (:744-781).
+ * rule 0: a node joins the aggregate of its first aggregated neighbour in
+ *   adjacency order -- what the reference kernel computes: its candidate
+ *   table fills all ten slots with the first candidate, so later candidates
+ *   are never counted;
+ * rule 1: the most frequent neighbouring aggregate, ties to the smallest id.
+ * Deviation: nodes that no aggregate can reach (a component emptied by runt
+ * removal) become singleton aggregates and the loop ends; the reference
+ * loops forever there.  Returns the number of aggregates. */
+int64_t orc_aggregate(int64_t n, const int32_t* ap, const int32_t* aj, int depth, int min_size, int rule,
+                      uint64_t seed, int32_t* agg) {
+  if (n == 0) return 0;
+  int32_t* mis = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* pin = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* pout = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* aggd = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  orc_mis(n, ap, aj, depth, seed, mis);
+  int32_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (mis[i] == 1) {
+      pin[i] = cnt++;
+      aggd[i] = 1;
+    } else {
+      pin[i] = -1;
+      aggd[i] = 0;
+    }
+  }
+  int64_t nagg = 0;
+  int64_t prev_unalloc = -1;
+  for (;;) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      if (aggd[i]) {
+        pout[i] = pin[i];
+        continue;
+      }
+      int32_t add = -1;
+      if (rule == 0) {
+        for (int32_t q = ap[i]; q < ap[i + 1]; ++q)
+          if (pin[aj[q]] != -1) {
+            add = pin[aj[q]];
+            break;
+          }
+      } else {
+        int best = 0;
+        for (int32_t q = ap[i]; q < ap[i + 1]; ++q) {
+          int32_t c = pin[aj[q]];
+          if (c == -1) continue;
+          int seen = 0;
+          for (int32_t r = ap[i]; r < q; ++r)
+            if (pin[aj[r]] == c) { seen = 1; break; }
+          if (seen) continue;
+          int k = 0;
+          for (int32_t r = q; r < ap[i + 1]; ++r) k += (pin[aj[r]] == c);
+          if (k > best || (k == best && c < add)) {
+            best = k;
+            add = c;
+          }
+        }
+      }
+      pout[i] = add;
+      if (add != -1) aggd[i] = 1;
+    }
+    int32_t* t = pin; pin = pout; pout = t;
+    int64_t unalloc = 0;
+    for (int64_t i = 0; i < n; ++i) unalloc += (aggd[i] == 0);
+    if (unalloc == 0) {
+      int32_t mx = -1;
+      for (int64_t i = 0; i < n; ++i) if (pin[i] > mx) mx = pin[i];
+      int64_t np = mx + 1;
+      int32_t* sz = (int32_t*)calloc((size_t)np, sizeof(int32_t));
+      int32_t* ex = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+      for (int64_t i = 0; i < n; ++i) sz[pin[i]]++;
+      int32_t removed = 0;
+      for (int64_t p = 0; p < np; ++p) {
+        ex[p] = removed;
+        removed += (sz[p] < min_size);
+      }
+      if (removed == 0) {
+        nagg = np;
+        free(sz); free(ex);
+        break;
+      }
+      for (int64_t i = 0; i < n; ++i) {
+        int32_t p = pin[i];
+        if (sz[p] < min_size) {
+          pin[i] = -1;
+          aggd[i] = 0;
+        } else {
+          pin[i] = p - ex[p];
+          aggd[i] = 1;
+        }
+      }
+      free(sz); free(ex);
+      prev_unalloc = -1;
+      continue;
+    }
+    if (unalloc == prev_unalloc) {
+      int32_t mx = -1;
+      for (int64_t i = 0; i < n; ++i) if (pin[i] > mx) mx = pin[i];
+      int32_t next = mx + 1;
+      for (int64_t i = 0; i < n; ++i) if (!aggd[i]) pin[i] = next++;
+      nagg = next;
+      break;
+    }
+    prev_unalloc = unalloc;
+  }
+  memcpy(agg, pin, sizeof(int32_t) * (size_t)n);
+  free(mis); free(pin); free(pout); free(aggd);
+  return nagg;
+}

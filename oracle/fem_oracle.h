@@ -20,3 +20,25 @@ void orc_element_grad(int dim, const double* xyz, int64_t ne, const int32_t* con
 }
 #endif
 #endif
+#ifdef __cplusplus
+extern "C" {
+#endif
+uint32_t orc_mis_seed(uint64_t seed, uint32_t i);
+void orc_mis(int64_t n, const int32_t* ap, const int32_t* aj, int depth, uint64_t seed, int32_t* mis);
+int64_t orc_aggregate(int64_t n, const int32_t* ap, const int32_t* aj, int depth, int min_size, int rule,
+                      uint64_t seed, int32_t* agg);
+#ifdef __cplusplus
+}
+#endif
+#ifdef __cplusplus
+extern "C" {
+#endif
+void orc_induced_graph(int64_t n, const int32_t* gp, const int32_t* gj, const int32_t* agg, int64_t nc, int32_t** cp,
+                       int32_t** cj);
+void orc_free(void* p);
+int orc_amg_pcg(int64_t n, const int64_t* rp, const int32_t* ci, const double* val, const double* b, double* x,
+                double tol, int max_iters, int top_size, int max_levels, int pre, int post, int relaxes,
+                double smoother_weight, double pro_omega, int part_max, uint64_t seed, double* out);
+#ifdef __cplusplus
+}
+#endif

@@ -677,6 +677,12 @@ void build_hierarchy(Handle& h) {
       }
     }
   }
+  // sliced-ELL copies for the solve phase (operators and prolongators)
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    Level& L = *H.levels[l];
+    build_sell((l == 0) ? h.A : L.A, H.precision == 1, s);
+    if ((int)l != H.coarse) build_sell(L.P, H.precision == 1, s);
+  }
   HIP_CHECK(hipStreamSynchronize(s));
   H.built = true;
   H.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -488,6 +488,7 @@ void build_mesh_device(Handle& h) {
 
 void assemble_values(Handle& h) {
   hipStream_t s = h.stream;
+  h.A.nslices = 0;  // the sliced-ELL copy is rebuilt with the next hierarchy
   const int npe = h.dim + 1;
   ProfScope ps(h, "assembly", (double)h.ne * (16 + 8) + (double)h.nv * 32 + (double)h.A.nnz * 12 +
                                   (double)h.ne * npe * 4);

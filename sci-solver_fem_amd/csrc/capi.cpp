@@ -153,6 +153,7 @@ void import_user_csr(Handle& h, const std::vector<int64_t>& rp, const std::vecto
   h.A.ci.upload(ici.data(), nnz, h.stream);
   h.A.v64.upload(iv.data(), nnz, h.stream);
   h.A.v32.release();
+  h.A.nslices = 0;
   HIP_CHECK(hipStreamSynchronize(h.stream));
 }
 

@@ -30,6 +30,14 @@ struct Csr {
   DevBuf<int32_t> ci;  // nnz
   DevBuf<double> v64;  // nnz
   DevBuf<float> v32;   // nnz (mixed precision only)
+  // Sliced ELL copy for the solve phase: slices of 64 consecutive rows (one
+  // wavefront, a lane per row), stored column-major inside the slice and
+  // padded to the longest row of the slice (padding: value 0, column = the row's last column).
+  int64_t nslices = 0;
+  DevBuf<int32_t> sp;   // nslices + 1 element offsets (multiples of 64)
+  DevBuf<int32_t> sc;   // padded columns
+  DevBuf<double> s64;   // padded FP64 values
+  DevBuf<float> s32;    // padded FP32 values (mixed precision only)
 };
 
 struct Level {
@@ -149,6 +157,8 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 // values of C = L * Rm (FP64) for the pattern found by spgemm_symbolic
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
+// (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when with_f32)
+void build_sell(Csr& A, bool with_f32, hipStream_t s);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----

@@ -85,6 +85,38 @@ __global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const 
   spmv_epilogue<MODE, T, V>(row * k + c, row, acc, y, x, b, dinv, d, xn, c1, c2);
 }
 
+// Sliced ELL, one RHS: a wavefront owns a slice of 64 rows, lane = row; the
+// slice is column-major so every step k is one coalesced 64-wide load of
+// values and of columns, and the k steps are independent loads the compiler
+// keeps in flight together.  Sequential per-row sum (fixed order).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ sp,
+                                                 const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                 const T* __restrict__ x, T* __restrict__ y,
+                                                 const T* __restrict__ b, const T* __restrict__ dinv,
+                                                 T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
+  const int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (slice >= nslices) return;
+  const int off = sp[slice];
+  const int w = (sp[slice + 1] - off) >> 6;
+  const int32_t* cp = sc + off + lane;
+  const T* vp = sv + off + lane;
+  T acc = T(0);
+  int k = 0;
+  for (; k + 4 <= w; k += 4) {
+    const int c0 = cp[64 * k], c1i = cp[64 * (k + 1)], c2i = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
+    const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+    acc += v0 * x[c0];
+    acc += v1 * x[c1i];
+    acc += v2 * x[c2i];
+    acc += v3 * x[c3];
+  }
+  for (; k < w; ++k) acc += vp[64 * k] * x[cp[64 * k]];
+  const int64_t row = slice * 64 + lane;
+  if (row < n) spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
+}
+
 // ---- reductions ---------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {

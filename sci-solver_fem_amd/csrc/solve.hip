@@ -13,6 +13,7 @@
 //  * the stop test is re-checked on the true residual b - A x;
 //  * scalars (alpha, beta, norms) stay on the device; the host reads one
 //    vector of squared residual norms per iteration.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -92,27 +93,40 @@ void launch_multi(const Csr& A, int k, const V* val, const T* x, T* y, const T* 
                                                        c1, c2);
 }
 
-// y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.
-template <typename V, typename T, int MODE>
-void spmv_op(const Csr& A, int k, const V* val, const T* x, T* y, const T* b, const V* dinv, T* d, T* xn, T c1, T c2,
+template <typename T>
+const T* svals_of(const Csr& A);
+template <>
+const double* svals_of<double>(const Csr& A) { return A.s64.get(); }
+template <>
+const float* svals_of<float>(const Csr& A) { return A.s32.get(); }
+
+// y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.  One RHS
+// uses the sliced-ELL copy when it exists, otherwise CSR with G lanes per row.
+template <typename T, int MODE>
+void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn, T c1, T c2,
              hipStream_t s) {
   if (A.n == 0) return;
-  if (k == 1) {
+  const T* val = vals_of<T>(A);
+  if (k == 1 && A.nslices > 0 && svals_of<T>(A)) {
+    unsigned grid = grid_for(A.nslices, kNT / 64);
+    sell_kernel<T, MODE><<<grid, kNT, 0, s>>>(A.n, A.nslices, A.sp.get(), A.sc.get(), svals_of<T>(A), x, y, b, dinv,
+                                              d, xn, c1, c2);
+  } else if (k == 1) {
     switch (lanes_for(A)) {
-      case 4: launch_vec<V, T, MODE, 4>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 8: launch_vec<V, T, MODE, 8>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 16: launch_vec<V, T, MODE, 16>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 32: launch_vec<V, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      default: launch_vec<V, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_vec<T, T, MODE, 4>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_vec<T, T, MODE, 8>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_vec<T, T, MODE, 16>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 32: launch_vec<T, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_vec<T, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
   } else {
     switch (pow2_at_least(k)) {
-      case 2: launch_multi<V, T, MODE, 2>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 4: launch_multi<V, T, MODE, 4>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 8: launch_multi<V, T, MODE, 8>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 16: launch_multi<V, T, MODE, 16>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      case 32: launch_multi<V, T, MODE, 32>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
-      default: launch_multi<V, T, MODE, 64>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 2: launch_multi<T, T, MODE, 2>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_multi<T, T, MODE, 4>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_multi<T, T, MODE, 8>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_multi<T, T, MODE, 16>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 32: launch_multi<T, T, MODE, 32>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_multi<T, T, MODE, 64>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
   }
   HIP_LAUNCH_CHECK();
@@ -177,6 +191,41 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
     if (lane == 0) {
       q[row] = acc;
       part += acc * p[row];
+    }
+  }
+  double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// q = A p on the sliced-ELL copy and per-block partials of p.q (one column)
+__global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ sp,
+                                                         const int32_t* __restrict__ sc,
+                                                         const double* __restrict__ sv, const double* __restrict__ p,
+                                                         double* __restrict__ q, double* __restrict__ partials) {
+  __shared__ double s_buf[kNT];
+  const int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  if (slice < nslices) {
+    const int off = sp[slice];
+    const int w = (sp[slice + 1] - off) >> 6;
+    const int32_t* cp = sc + off + lane;
+    const double* vp = sv + off + lane;
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+      const int c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
+      const double v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+      acc += v0 * p[c0];
+      acc += v1 * p[c1];
+      acc += v2 * p[c2];
+      acc += v3 * p[c3];
+    }
+    for (; k < w; ++k) acc += vp[64 * k] * p[cp[64 * k]];
+    const int64_t row = slice * 64 + lane;
+    if (row < n) {
+      q[row] = acc;
+      part = acc * p[row];
     }
   }
   double tot = block_sum(part, s_buf);
@@ -315,12 +364,12 @@ template <typename T>
 T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, bool zero, int k) {
   Level& L = *h.H.levels[l];
   hipStream_t s = h.stream;
-  const T* val = vals_of<T>(A);
   const T* dinv = dinv_of<T>(L);
   Smoother S = smoother_for(h, L);
   const bool prof = h.profiling && l == 0;
-  const double sweep_bytes = (double)A.nnz * (sizeof(T) + 4) + 4.0 * (A.n + 1) +
-                             (double)A.n * (sizeof(T) * (S.type ? 6.0 * k : 3.0 * k) + sizeof(T));
+  // algorithmic bytes of one sweep: matrix (value + column per nonzero) and the
+  // per-row vectors (Jacobi: x, b, dinv, x_new; Chebyshev: + d read and write)
+  const double sweep_bytes = (double)A.nnz * (sizeof(T) + 4) + (double)A.n * sizeof(T) * (S.type ? 6.0 : 4.0) * k;
   if (nu <= 0) {
     if (zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(T) * A.n * k, s));
     return x;
@@ -337,7 +386,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
     }
     for (; step < nu; ++step) {
       ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
-      spmv_op<T, T, kJacobi>(A, k, val, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
+      spmv_op<T, kJacobi>(A, k, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
       T* tmp = cur; cur = nxt; nxt = tmp;
     }
     return cur;
@@ -363,7 +412,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
       rho_old = rho_new;
     }
     ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
-    spmv_op<T, T, kCheb>(A, k, val, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
+    spmv_op<T, kCheb>(A, k, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
     T* tmp = cur; cur = nxt; nxt = tmp;
   }
   return cur;
@@ -391,15 +440,15 @@ T* vcycle(Handle& h, int l, const T* b, int k) {
   T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k);
   T* oth = (cur == x) ? t : x;
   {
-    ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", (double)A.nnz * (sizeof(T) + 4) + 4.0 * (A.n + 1) +
+    ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", (double)A.nnz * (sizeof(T) + 4) +
                                                                  3.0 * A.n * k * sizeof(T));
-    spmv_op<T, T, kResid>(A, k, vals_of<T>(A), cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+    spmv_op<T, kResid>(A, k, cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   }
   Level& N = *H.levels[l + 1];
   T* bc = LV<T>::b(N);
-  spmv_op<T, T, kAssign>(L.R, k, vals_of<T>(L.R), r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+  spmv_op<T, kAssign>(L.R, k, r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   T* xc = vcycle<T>(h, l + 1, bc, k);
-  spmv_op<T, T, kAdd>(L.P, k, vals_of<T>(L.P), xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
+  spmv_op<T, kAdd>(L.P, k, xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k);
   return cur;
 }
@@ -479,13 +528,11 @@ void release_profile_events(Handle& h) {
 }
 
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s) {
-  spmv_op<double, double, kAssign>(A, 1, A.v64.get(), x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0,
-                                   0.0, s);
+  spmv_op<double, kAssign>(A, 1, x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
 }
 
 void apply_operator_int(Handle& h, const double* x, double* y, int k) {
-  spmv_op<double, double, kAssign>(h.A, k, h.A.v64.get(), x, y, nullptr, (const double*)nullptr, nullptr, nullptr,
-                                   0.0, 0.0, h.stream);
+  spmv_op<double, kAssign>(h.A, k, x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, h.stream);
 }
 
 void ensure_krylov(Handle& h, int k) {
@@ -494,7 +541,7 @@ void ensure_krylov(Handle& h, int k) {
     h.kx.resize(n * k); h.kr.resize(n * k); h.kp.resize(n * k); h.kq.resize(n * k);
     h.kz.resize(n * k); h.kb.resize(n * k);
     h.ubuf.resize(n * k); h.ubuf2.resize(n * k);
-    h.partials.resize((size_t)4096 * 64);
+    h.partials.resize((size_t)std::max<int64_t>(4096, n / 256 + 2) * 64);
     h.scal.resize((size_t)S_COUNT * 64);
     if (!h.h_scal) HIP_CHECK(hipHostMalloc((void**)&h.h_scal, sizeof(double) * S_COUNT * 64, hipHostMallocDefault));
     h.kcap = k;
@@ -565,6 +612,15 @@ void update(Handle& h, int k, double tol2) {
   }
 }
 
+void launch_sell_dot(Handle& h) {
+  const Csr& A = h.A;
+  unsigned g = grid_for(A.nslices, kNT / 64);
+  pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, A.sp.get(), A.sc.get(), A.s64.get(), h.kp.get(),
+                                               h.kq.get(), h.partials.get());
+  HIP_LAUNCH_CHECK();
+  finish(h, (int)g, 1, F_ALPHA, 0);
+}
+
 template <int G>
 void launch_spmv_dot(Handle& h) {
   const Csr& A = h.A;
@@ -589,8 +645,10 @@ void launch_spmm_dot(Handle& h, int k) {
 
 void spmv_dot(Handle& h, int k) {
   const Csr& A = h.A;
-  ProfScope ps(h, "pcg_spmv", (double)A.nnz * 12 + 4.0 * (A.n + 1) + 3.0 * 8 * A.n * k);
-  if (k == 1) {
+  ProfScope ps(h, "pcg_spmv", (double)A.nnz * 12 + 3.0 * 8 * A.n * k);
+  if (k == 1 && A.nslices > 0 && A.s64.size()) {
+    launch_sell_dot(h);
+  } else if (k == 1) {
     switch (lanes_for(A)) {
       case 4: launch_spmv_dot<4>(h); break;
       case 8: launch_spmv_dot<8>(h); break;
@@ -643,8 +701,8 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
   int status = FEM_OK;
   auto true_residual = [&](std::vector<double>& rel) {
     // kq = b - A x ; rel_c = ||kq_c|| / ||b_c||
-    spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kq.get(), h.kb.get(), (const double*)nullptr,
-                                    nullptr, nullptr, 0.0, 0.0, s);
+    spmv_op<double, kResid>(h.A, k, h.kx.get(), h.kq.get(), h.kb.get(), (const double*)nullptr, nullptr, nullptr,
+                            0.0, 0.0, s);
     dot(h, k, h.kq.get(), h.kq.get(), F_STORE, S_TRUE);
     std::vector<double> t(k);
     get_scal(h, S_TRUE, k, t.data());
@@ -662,8 +720,8 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       if (rel[c] > tol) status = FEM_ERR_NOT_CONVERGED;
   } else {
     // r = b - A x0
-    spmv_op<double, double, kResid>(h.A, k, h.A.v64.get(), h.kx.get(), h.kr.get(), h.kb.get(),
-                                    (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
+    spmv_op<double, kResid>(h.A, k, h.kx.get(), h.kr.get(), h.kb.get(), (const double*)nullptr, nullptr, nullptr,
+                            0.0, 0.0, s);
     dot(h, k, h.kr.get(), h.kr.get(), F_STORE, S_RR);
     get_scal(h, S_RR, k, rr.data());
     bool any = false;

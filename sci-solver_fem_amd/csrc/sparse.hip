@@ -245,6 +245,49 @@ __global__ void transpose_ptr_kernel(int64_t m, const uint64_t* __restrict__ key
   rp[c] = (int32_t)lo;
 }
 
+__global__ void sell_width_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp, int32_t* __restrict__ w) {
+  int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslices) return;
+  int mx = 0;
+  for (int64_t r = sl * 64; r < sl * 64 + 64 && r < n; ++r) mx = max(mx, rp[r + 1] - rp[r]);
+  w[sl] = mx * 64;
+}
+
+// one thread per (slice, lane): row = slice * 64 + lane
+__global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci, const double* __restrict__ v,
+                                 const int32_t* __restrict__ sp, int32_t* __restrict__ sc,
+                                 double* __restrict__ s64, float* __restrict__ s32) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslices * 64) return;
+  int64_t sl = t >> 6;
+  int lane = (int)(t & 63);
+  int64_t row = t;
+  int off = sp[sl];
+  int w = (sp[sl + 1] - off) >> 6;
+  int len = 0, base = 0;
+  if (row < n) {
+    base = rp[row];
+    len = rp[row + 1] - base;
+  }
+  // padding repeats the row's last column (a valid index for rectangular P
+  // too, and already in cache); empty rows and rows past n use column 0
+  int pad_col = len > 0 ? ci[base + len - 1] : 0;
+  for (int k = 0; k < w; ++k) {
+    int64_t e = off + (int64_t)k * 64 + lane;
+    if (k < len) {
+      sc[e] = ci[base + k];
+      double val = v[base + k];
+      s64[e] = val;
+      if (s32) s32[e] = (float)val;
+    } else {
+      sc[e] = pad_col;
+      s64[e] = 0.0;
+      if (s32) s32[e] = 0.0f;
+    }
+  }
+}
+
 int bits_for(int64_t v) {
   int b = 1;
   while (b < 63 && ((int64_t)1 << b) <= v) ++b;
@@ -315,6 +358,28 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
                                                             Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(),
                                                             C.v64.get());
   HIP_LAUNCH_CHECK();
+}
+
+void build_sell(Csr& A, bool with_f32, hipStream_t s) {
+  const int64_t ns = (A.n + 63) / 64;
+  A.nslices = ns;
+  A.sp.resize(ns + 1);
+  DevBuf<int32_t> w(ns > 0 ? ns : 1);
+  if (ns > 0) {
+    sell_width_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), w.get());
+    HIP_LAUNCH_CHECK();
+  }
+  int64_t tot = dev_exclusive_scan_i32(w.get(), A.sp.get(), ns, s);
+  if (tot >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "sliced-ELL copy has 2^31 or more entries");
+  A.sc.resize(tot > 0 ? tot : 1);
+  A.s64.resize(tot > 0 ? tot : 1);
+  if (with_f32) A.s32.resize(tot > 0 ? tot : 1); else A.s32.release();
+  if (ns > 0) {
+    sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), A.sp.get(),
+                                                            A.sc.get(), A.s64.get(), with_f32 ? A.s32.get() : nullptr);
+    HIP_LAUNCH_CHECK();
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s) {

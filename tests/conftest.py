@@ -69,9 +69,17 @@ class Oracle:
         lib.orc_gauss_lobatto_jacobi.argtypes = [C.c_int, C.c_double, C.c_double, V, V]
         lib.orc_gauss_radau_jacobi.argtypes = [C.c_int, C.c_double, C.c_double, V, V]
         lib.orc_element_grad.argtypes = [C.c_int, V, C.c_int64, V, V, V, V]
-        for name in ("orc_mis", "orc_aggregate", "orc_amg_pcg"):
-            if hasattr(lib, name):
-                pass
+        lib.orc_mis_seed.restype = C.c_uint32
+        lib.orc_mis_seed.argtypes = [C.c_uint64, C.c_uint32]
+        lib.orc_mis.argtypes = [C.c_int64, V, V, C.c_int, C.c_uint64, V]
+        lib.orc_aggregate.restype = C.c_int64
+        lib.orc_aggregate.argtypes = [C.c_int64, V, V, C.c_int, C.c_int, C.c_int, C.c_uint64, V]
+        lib.orc_induced_graph.argtypes = [C.c_int64, V, V, V, C.c_int64, C.POINTER(C.POINTER(C.c_int32)),
+                                          C.POINTER(C.POINTER(C.c_int32))]
+        lib.orc_free.argtypes = [V]
+        lib.orc_amg_pcg.restype = C.c_int
+        lib.orc_amg_pcg.argtypes = [C.c_int64, V, V, V, V, V, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, V]
 
     @staticmethod
     def _p(a):
@@ -138,6 +146,55 @@ class Oracle:
         fn = self.lib.orc_gauss_lobatto_jacobi if kind == "lobatto" else self.lib.orc_gauss_radau_jacobi
         fn(q, float(a), float(b), z.ctypes.data, w.ctypes.data)
         return z, w
+
+    def mis(self, ap, aj, depth, seed):
+        ap = np.ascontiguousarray(ap, dtype=np.int32)
+        aj = np.ascontiguousarray(aj, dtype=np.int32)
+        out = np.empty(ap.size - 1, dtype=np.int32)
+        self.lib.orc_mis(ap.size - 1, ap.ctypes.data, aj.ctypes.data, depth, seed, out.ctypes.data)
+        return out
+
+    def aggregate(self, ap, aj, depth=2, min_size=9, rule=0, seed=0x5EED):
+        ap = np.ascontiguousarray(ap, dtype=np.int32)
+        aj = np.ascontiguousarray(aj, dtype=np.int32)
+        out = np.empty(ap.size - 1, dtype=np.int32)
+        na = self.lib.orc_aggregate(ap.size - 1, ap.ctypes.data, aj.ctypes.data, depth, min_size, rule, seed,
+                                    out.ctypes.data)
+        return out, int(na)
+
+    def induced_graph(self, ap, aj, agg, nc):
+        ap = np.ascontiguousarray(ap, dtype=np.int32)
+        aj = np.ascontiguousarray(aj, dtype=np.int32)
+        agg = np.ascontiguousarray(agg, dtype=np.int32)
+        cp = C.POINTER(C.c_int32)()
+        cj = C.POINTER(C.c_int32)()
+        self.lib.orc_induced_graph(ap.size - 1, ap.ctypes.data, aj.ctypes.data, agg.ctypes.data, nc, C.byref(cp),
+                                   C.byref(cj))
+        P = np.ctypeslib.as_array(cp, shape=(nc + 1,)).copy()
+        J = np.ctypeslib.as_array(cj, shape=(max(int(P[-1]), 1),))[:P[-1]].copy()
+        self.lib.orc_free(C.cast(cp, C.c_void_p))
+        self.lib.orc_free(C.cast(cj, C.c_void_p))
+        return P, J
+
+    def amg_pcg(self, xyz, conn, kappa, mass, b, tol=1e-8, max_iters=500, top_size=256, max_levels=100, pre=5,
+                post=5, relaxes=1, smoother_weight=1.0, pro_omega=0.67, part_max=512, seed=0x5EED, A=None):
+        """Reference AMG-PCG on the host; returns a dict with x and timings."""
+        if A is None:
+            rp, ci, v = self.assemble(xyz, conn, kappa, mass)
+        else:
+            rp, ci, v = A
+            rp = np.ascontiguousarray(rp, dtype=np.int64)
+            ci = np.ascontiguousarray(ci, dtype=np.int32)
+            v = np.ascontiguousarray(v, dtype=np.float64)
+        n = rp.size - 1
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(n)
+        out = np.zeros(8)
+        st = self.lib.orc_amg_pcg(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, bb.ctypes.data, x.ctypes.data,
+                                  tol, max_iters, top_size, max_levels, pre, post, relaxes, smoother_weight,
+                                  pro_omega, part_max, seed, out.ctypes.data)
+        return dict(x=x, status=st, iterations=int(out[0]), setup_s=out[1], solve_s=out[2], rel_residual=out[3],
+                    threads=int(out[4]), levels=int(out[5]), operator_complexity=out[6], converged=bool(out[7]))
 
     def element_grad(self, xyz, conn, a, u):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
