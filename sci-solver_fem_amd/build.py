@@ -57,9 +57,21 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     for o in objs:
         o.unlink()
+    build_examples()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def build_examples() -> None:
+    """C++ drivers with the reference examples' command lines (link the C ABI)."""
+    inc = PKG.parent / "include"
+    for ex in ("example1", "example2"):
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", str(inc), str(PKG / "examples" / f"{ex}.cpp"),
+               "-L", str(OUT), "-lfem_mi355x", "-Wl,-rpath,$ORIGIN", "-o", str(OUT / ex)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

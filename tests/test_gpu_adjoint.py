@@ -1,0 +1,111 @@
+"""Adjoint gradient dL/dkappa (no reference counterpart): against central finite
+differences along seeded random directions and against an FP64 CPU gradient
+(SuperLU solves on the oracle's matrix + the oracle's element gradient).
+
+Loss L(kappa) = 0.5 ||u(kappa) - u_obs||^2, K(kappa) u = b,
+dL/dkappa_e = -|T_e| grad(lambda) . grad(u) with K lambda = u - u_obs."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from conftest import report
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(meshgen, N=9, seed=3):
+    xyz, tets = meshgen.cube_mesh(N, seed=seed)
+    kappa = meshgen.lognormal_kappa(seed, tets.shape[0], 1.0)
+    kappa2 = meshgen.lognormal_kappa(seed + 101, tets.shape[0], 1.0)
+    b = meshgen.gaussian_rhs(seed, xyz.shape[0])
+    return xyz, tets, kappa, kappa2, b
+
+
+def _tight(s, precision=0):
+    # 1e-12: the FP64 true residual of these systems floors near 1e-14..1e-13
+    s.set_params(solverType=1, tolerance=1e-12, maxIters=1000, precision=precision, smootherType=1,
+                 preInnerIters=2, postInnerIters=2, topSize=64)
+
+
+def test_adjoint_matches_fp64_cpu_gradient(fem, gpu, oracle, meshgen):
+    report("[gpu] adjoint vs FP64 CPU gradient N=9")
+    xyz, tets, kappa, kappa2, b = _setup(meshgen)
+    A2 = oracle.scipy_matrix(xyz, tets, kappa2, 1e-2)
+    u_obs = spla.spsolve(A2.tocsc(), b) + meshgen.noise(7, xyz.shape[0], 1e-3)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    _tight(s)
+    u, info = s.solve(b)
+    assert info["status"] == 0
+    g, lam, ainfo = s.adjoint_grad(u, u - u_obs)
+    assert ainfo["status"] == 0
+    # FP64 CPU reference
+    A = oracle.scipy_matrix(xyz, tets, kappa, 1e-2)
+    lu = spla.splu(A.tocsc())
+    u_ref = lu.solve(b)
+    lam_ref = lu.solve(u_ref - u_obs)
+    g_ref = oracle.element_grad(xyz, tets, lam_ref, u_ref)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    # error bound ~ cond(K) x tolerance for u and lambda
+    assert np.linalg.norm(lam - lam_ref) / np.linalg.norm(lam_ref) < 1e-7
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-7
+    s.close()
+
+
+def test_adjoint_matches_finite_differences(fem, gpu, oracle, meshgen):
+    """GPU adjoint gradient against central differences of the loss computed
+    with FP64 direct solves (h = 1e-6, 16 seeded directions)."""
+    report("[gpu] adjoint vs central differences (16 directions, h=1e-6)")
+    xyz, tets, kappa, kappa2, b = _setup(meshgen, N=7, seed=5)
+    A2 = oracle.scipy_matrix(xyz, tets, kappa2, 1e-2)
+    u_obs = spla.spsolve(A2.tocsc(), b) + meshgen.noise(9, xyz.shape[0], 1e-3)
+
+    def loss(k):
+        u = spla.splu(oracle.scipy_matrix(xyz, tets, k, 1e-2).tocsc()).solve(b)
+        return 0.5 * float(np.dot(u - u_obs, u - u_obs))
+
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    _tight(s)
+    u, _ = s.solve(b)
+    g, _, info = s.adjoint_grad(u, u - u_obs)
+    assert info["status"] == 0
+    rng = np.random.default_rng(16)
+    h = 1e-6
+    errs = []
+    for _ in range(16):
+        d = rng.normal(size=kappa.size) * kappa  # relative perturbation keeps kappa > 0
+        fd = (loss(kappa + h * d) - loss(kappa - h * d)) / (2 * h)
+        ad = float(np.dot(g, d))
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-30))
+    report(f"      max rel FD error {max(errs):.2e}")
+    assert max(errs) < 1e-5
+    s.close()
+
+
+def test_torch_autograd_function(fem, gpu, oracle, meshgen):
+    import torch
+    report("[gpu] torch.autograd.Function backward")
+    xyz, tets, kappa, kappa2, b = _setup(meshgen, N=7, seed=11)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    _tight(s, precision=1)
+    dev = torch.device("cuda:0")
+    k_t = torch.tensor(kappa, dtype=torch.float64, device=dev, requires_grad=True)
+    b_t = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
+    u_obs = torch.tensor(meshgen.noise(3, xyz.shape[0], 1.0), dtype=torch.float64, device=dev)
+    Fn = fem.FEMSolveFunction
+    u = Fn.apply(k_t, b_t, s)
+    L = 0.5 * ((u - u_obs) ** 2).sum()
+    L.backward()
+    # FP64 CPU reference of both gradients
+    A = oracle.scipy_matrix(xyz, tets, kappa, 1e-2)
+    lu = spla.splu(A.tocsc())
+    u_ref = lu.solve(b)
+    lam = lu.solve(u_ref - u_obs.cpu().numpy())
+    g_ref = oracle.element_grad(xyz, tets, lam, u_ref)
+    gk = k_t.grad.cpu().numpy()
+    gb = b_t.grad.cpu().numpy()
+    assert np.linalg.norm(u.detach().cpu().numpy() - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    assert np.linalg.norm(gk - g_ref) / np.linalg.norm(g_ref) < 1e-7
+    assert np.linalg.norm(gb - lam) / np.linalg.norm(lam) < 1e-7
+    s.close()
