@@ -177,6 +177,7 @@ class FEMSolver {
   int smootherType_;
   int aggregationRule_;
   unsigned long long seed_;
+  double strengthThreshold_;
 
  private:
   void load_fields(const fem_params& p) {
@@ -203,6 +204,7 @@ class FEMSolver {
     smootherType_ = p.smootherType;
     aggregationRule_ = p.aggregationRule;
     seed_ = p.seed;
+    strengthThreshold_ = p.strengthThreshold;
   }
   void push_fields() {
     fem_params p;
@@ -230,6 +232,7 @@ class FEMSolver {
     p.smootherType = smootherType_;
     p.aggregationRule = aggregationRule_;
     p.seed = seed_;
+    p.strengthThreshold = strengthThreshold_;
     if (fem_set_params(h_, &p) != FEM_OK) throw std::invalid_argument(std::string("FEMSolver: ") + fem_last_error(h_));
   }
 

@@ -74,6 +74,10 @@ typedef struct fem_params {
                              aggregate, ties to the lowest aggregate id       */
   int coarseSolve;        /* 0: FP64 Cholesky-based dense inverse on device   */
   unsigned long long seed;/* seed of the randomized MIS (reference: time())  */
+  double strengthThreshold;/* 0: aggregate the full pattern at level 0 and the
+                             induced aggregate graph below (the reference);
+                             > 0: aggregate the strength graph of every level,
+                             |a_ij| >= theta sqrt(a_ii a_jj)                   */
 } fem_params;
 
 /* Fill *p with the reference defaults (src/FEMSolver.cu:11-33). */

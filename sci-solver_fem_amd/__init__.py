@@ -68,7 +68,7 @@ _FIELDS = {
     "device_": "device", "blockSize_": "blockSize",
     # additions
     "precision_": "precision", "smootherType_": "smootherType",
-    "aggregationRule_": "aggregationRule", "seed_": "seed",
+    "aggregationRule_": "aggregationRule", "seed_": "seed", "strengthThreshold_": "strengthThreshold",
 }
 
 

@@ -48,6 +48,7 @@ class FemParams(C.Structure):
         ("aggregationRule", C.c_int),
         ("coarseSolve", C.c_int),
         ("seed", C.c_ulonglong),
+        ("strengthThreshold", C.c_double),
     ]
 
 

@@ -27,6 +27,8 @@ namespace {
 
 constexpr int kMisThreads = 32768;  // Generate_Randoms_Kernel: 128 x 256 generator threads
 constexpr int kMisInMis = 1000001;
+// largest coarsest level inverted densely on the host (5n^3/6 multiply-adds)
+constexpr int64_t kMaxCoarse = 2048;
 
 inline uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -405,7 +407,14 @@ void aggregate_members(const int32_t* agg, int64_t n, int64_t nagg, Csr& M, hipS
 // Cholesky breaks down).  a is row-major n x n, replaced by its inverse.
 bool dense_inverse(std::vector<double>& a, int64_t n) {
   std::vector<double> L(a);
+  // Cholesky reads the lower triangle only: take it for symmetric matrices
+  // (Galerkin products are symmetric to rounding), LU for the rest
+  double amax = 0.0;
+  for (int64_t i = 0; i < n * n; ++i) amax = std::max(amax, std::fabs(a[i]));
   bool spd = true;
+  for (int64_t i = 0; i < n && spd; ++i)
+    for (int64_t j = 0; j < i; ++j)
+      if (std::fabs(a[i * n + j] - a[j * n + i]) > 1e-12 * amax) { spd = false; break; }
   for (int64_t j = 0; j < n && spd; ++j) {
     double d = L[j * n + j];
     for (int64_t k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
@@ -420,18 +429,26 @@ bool dense_inverse(std::vector<double>& a, int64_t n) {
   }
   std::vector<double> inv(n * n, 0.0);
   if (spd) {
-    // solve L L^T X = I column by column
-    std::vector<double> y(n);
-    for (int64_t c = 0; c < n; ++c) {
-      for (int64_t i = 0; i < n; ++i) {
-        double v = (i == c) ? 1.0 : 0.0;
-        for (int64_t k = 0; k < i; ++k) v -= L[i * n + k] * y[k];
-        y[i] = v / L[i * n + i];
+    // W = L^-1 row by row, then A^-1 = W^T W; every inner loop runs along a
+    // row (unit stride), about 5n^3/6 multiply-adds in all
+    std::vector<double> W(n * n, 0.0), r(n);
+    for (int64_t i = 0; i < n; ++i) {
+      std::fill(r.begin(), r.begin() + i, 0.0);
+      for (int64_t k = 0; k < i; ++k) {
+        const double lik = L[i * n + k];
+        const double* wk = &W[k * n];
+        for (int64_t j = 0; j <= k; ++j) r[j] += lik * wk[j];
       }
-      for (int64_t i = n - 1; i >= 0; --i) {
-        double v = y[i];
-        for (int64_t k = i + 1; k < n; ++k) v -= L[k * n + i] * inv[k * n + c];
-        inv[i * n + c] = v / L[i * n + i];
+      const double di = 1.0 / L[i * n + i];
+      for (int64_t j = 0; j < i; ++j) W[i * n + j] = -r[j] * di;
+      W[i * n + i] = di;
+    }
+    for (int64_t k = 0; k < n; ++k) {
+      const double* wk = &W[k * n];
+      for (int64_t i = 0; i <= k; ++i) {
+        const double wki = wk[i];
+        double* row = &inv[i * n];
+        for (int64_t j = 0; j <= k; ++j) row[j] += wki * wk[j];
       }
     }
     a.swap(inv);
@@ -471,6 +488,44 @@ bool dense_inverse(std::vector<double>& a, int64_t n) {
   }
   a.swap(inv);
   return true;
+}
+
+// Strength graph: j != i with |a_ij| >= theta sqrt(a_ii a_jj) (1/dinv = a_ii).
+template <bool kFill>
+__global__ void strength_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                const double* __restrict__ v, const double* __restrict__ dinv, double theta,
+                                int32_t* __restrict__ cnt, const int32_t* __restrict__ gp, int32_t* __restrict__ gj) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double di = dinv[i];
+  int c = 0;
+  int o = kFill ? gp[i] : 0;
+  for (int q = rp[i]; q < rp[i + 1]; ++q) {
+    int j = ci[q];
+    if (j == (int)i) continue;
+    // |a_ij|^2 >= theta^2 a_ii a_jj  <=>  |a_ij|^2 dinv_i dinv_j >= theta^2
+    double a = v[q];
+    if (a * a * di * dinv[j] >= theta * theta) {
+      if (kFill) gj[o + c] = j;
+      ++c;
+    }
+  }
+  if (!kFill) cnt[i] = c;
+}
+
+void strength_graph(const Csr& A, const double* dinv, double theta, Csr& G, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<int32_t> cnt(n > 0 ? n : 1);
+  strength_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta,
+                                                          cnt.get(), nullptr, nullptr);
+  HIP_LAUNCH_CHECK();
+  G.n = G.m = n;
+  G.rp.resize(n + 1);
+  G.nnz = dev_exclusive_scan_i32(cnt.get(), G.rp.get(), n, s);
+  G.ci.resize(G.nnz > 0 ? G.nnz : 1);
+  strength_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta,
+                                                         nullptr, G.rp.get(), G.ci.get());
+  HIP_LAUNCH_CHECK();
 }
 
 void to_f32(Csr& A, hipStream_t s) {
@@ -600,17 +655,27 @@ void build_hierarchy(Handle& h) {
     }
     const bool last = (n < h.prm.topSize) || (l + 1 >= h.prm.maxLevels) || n <= 1;
     int64_t nagg = 0;
+    const double theta = h.prm.strengthThreshold;
+    if (!last && theta > 0.0) strength_graph(A, L.dinv64.get(), theta, G, s);
     if (!last) {
       L.agg.resize(n);
       unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
       aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
                       &nagg, s);
+      if (theta > 0.0 && 2 * nagg > n) {
+        // The strength graph fell apart into small components (each left as
+        // singletons after runt removal): coarsening would stall for many
+        // levels.  Aggregate this level on the full pattern instead.
+        spgemm_symbolic(A, nullptr, nullptr, 0, nullptr, n, true, G, s);
+        aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
+                        &nagg, s);
+      }
     }
     if (last || nagg >= n || nagg == 0) {
       // coarsest level: dense FP64 inverse
-      if (n > 8192)
-        fail(FEM_ERR_CAPACITY, "coarsest level has " + std::to_string(n) +
-                                   " rows (limit 8192); raise maxLevels or check the aggregation");
+      if (n > kMaxCoarse)
+        fail(FEM_ERR_CAPACITY, "coarsest level has " + std::to_string(n) + " rows (limit " +
+                                   std::to_string(kMaxCoarse) + "); raise maxLevels or lower topSize");
       L.agg.release();
       std::vector<int32_t> rp(n + 1), ci(A.nnz);
       std::vector<double> v(A.nnz);
@@ -651,8 +716,9 @@ void build_hierarchy(Handle& h) {
       spgemm_symbolic(L.R, AP.rp.get(), AP.ci.get(), AP.n, nullptr, nagg, false, next->A, s);
       spgemm_numeric(L.R, AP, next->A, s);
     }
-    // next graph: induced aggregate graph (aggregates sharing an edge)
-    {
+    // next graph: induced aggregate graph (aggregates sharing an edge); with a
+    // strength threshold the next level builds its own strength graph instead
+    if (theta <= 0.0) {
       Csr M;
       aggregate_members(L.agg.get(), n, nagg, M, s);
       Csr G2;
@@ -677,11 +743,24 @@ void build_hierarchy(Handle& h) {
       }
     }
   }
-  // sliced-ELL copies for the solve phase (operators and prolongators)
+  // Sliced-ELL copies for the solve phase, for operators with short rows (the
+  // fine level, prolongators).  Coarse operators have long rows (~50-300) for
+  // which a lane per row serialises; they keep the multi-lane CSR kernels.
+  auto sell_or_csr = [&](Csr& M) {
+    if (M.n > 0 && (double)M.nnz / (double)M.n <= 24.0) {
+      build_sell(M, H.precision == 1, s);
+    } else {
+      M.nslices = 0;
+      M.sp.release();
+      M.sc.release();
+      M.s64.release();
+      M.s32.release();
+    }
+  };
   for (size_t l = 0; l < H.levels.size(); ++l) {
     Level& L = *H.levels[l];
-    build_sell((l == 0) ? h.A : L.A, H.precision == 1, s);
-    if ((int)l != H.coarse) build_sell(L.P, H.precision == 1, s);
+    sell_or_csr((l == 0) ? h.A : L.A);
+    if ((int)l != H.coarse) sell_or_csr(L.P);
   }
   HIP_CHECK(hipStreamSynchronize(s));
   H.built = true;

@@ -160,7 +160,7 @@ void import_user_csr(Handle& h, const std::vector<int64_t>& rp, const std::vecto
 bool params_shape_changed(const fem_params& a, const fem_params& b) {
   return a.maxLevels != b.maxLevels || a.topSize != b.topSize || a.randMisParameters != b.randMisParameters ||
          a.proOmega != b.proOmega || a.precision != b.precision || a.aggregationRule != b.aggregationRule ||
-         a.seed != b.seed || a.aggregatorType != b.aggregatorType;
+         a.seed != b.seed || a.aggregatorType != b.aggregatorType || a.strengthThreshold != b.strengthThreshold;
 }
 
 void ensure_setup(Handle& h) {
@@ -232,6 +232,7 @@ void fem_params_default(fem_params* p) {
   p->aggregationRule = 0;
   p->coarseSolve = 0;
   p->seed = 0x5EEDull;
+  p->strengthThreshold = 0.0;
 }
 
 int fem_create_from_arrays(int dim, int64_t nv, const double* xyz, int64_t ne, const int32_t* conn,
@@ -293,7 +294,8 @@ int fem_set_params(fem_handle* hh, const fem_params* p) {
     Handle& h = *H(hh);
     if (p->maxIters < 0 || p->topSize < 1 || p->maxLevels < 1 || p->preInnerIters < 0 || p->postInnerIters < 0 ||
         !(p->tolerance >= 0) || p->precision < 0 || p->precision > 1 || p->smootherType < 0 ||
-        p->smootherType > 1 || p->aggregationRule < 0 || p->aggregationRule > 1)
+        p->smootherType > 1 || p->aggregationRule < 0 || p->aggregationRule > 1 || !(p->strengthThreshold >= 0) ||
+        p->strengthThreshold >= 1.0)
       fail(FEM_ERR_ARG, "invalid solver parameter");
     if (p->aggregatorType != 0) fail(FEM_ERR_ARG, "only aggregatorType 0 (randomized MIS) is implemented");
     if (params_shape_changed(h.prm, *p)) h.H.built = false;

@@ -197,16 +197,17 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// q = A p on the sliced-ELL copy and per-block partials of p.q (one column)
+// q = A p on the sliced-ELL copy and per-block partials of p.q (one column);
+// blocks stride over groups of 4 slices so the partial count stays <= grid.
 __global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ sp,
                                                          const int32_t* __restrict__ sc,
                                                          const double* __restrict__ sv, const double* __restrict__ p,
                                                          double* __restrict__ q, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
-  const int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double part = 0.0;
-  if (slice < nslices) {
+  for (int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6); slice < nslices;
+       slice += (int64_t)gridDim.x * (kNT / 64)) {
     const int off = sp[slice];
     const int w = (sp[slice + 1] - off) >> 6;
     const int32_t* cp = sc + off + lane;
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t ns
     const int64_t row = slice * 64 + lane;
     if (row < n) {
       q[row] = acc;
-      part = acc * p[row];
+      part += acc * p[row];
     }
   }
   double tot = block_sum(part, s_buf);
@@ -614,7 +615,7 @@ void update(Handle& h, int k, double tol2) {
 
 void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
-  unsigned g = grid_for(A.nslices, kNT / 64);
+  unsigned g = std::min<unsigned>(grid_for(A.nslices, kNT / 64), 2048u);
   pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, A.sp.get(), A.sc.get(), A.s64.get(), h.kp.get(),
                                                h.kq.get(), h.partials.get());
   HIP_LAUNCH_CHECK();

@@ -71,15 +71,18 @@ def test_adjoint_matches_finite_differences(fem, gpu, oracle, meshgen):
     g, _, info = s.adjoint_grad(u, u - u_obs)
     assert info["status"] == 0
     rng = np.random.default_rng(16)
-    h = 1e-6
-    errs = []
-    for _ in range(16):
-        d = rng.normal(size=kappa.size) * kappa  # relative perturbation keeps kappa > 0
-        fd = (loss(kappa + h * d) - loss(kappa - h * d)) / (2 * h)
-        ad = float(np.dot(g, d))
-        errs.append(abs(fd - ad) / max(abs(ad), 1e-30))
-    report(f"      max rel FD error {max(errs):.2e}")
-    assert max(errs) < 1e-5
+    dirs = [rng.normal(size=kappa.size) * kappa for _ in range(16)]  # relative: kappa stays > 0
+    # h = 1e-6: FD rounding (~1e-12 L / h) bounds the agreement near 1e-4;
+    # h = 1e-4: truncation ~h^2, rounding 100x smaller -> a much tighter check
+    # error scaled by |g| |d| (directional derivatives can cancel to ~0)
+    for h, tol in ((1e-6, 1e-4), (1e-4, 1e-6)):
+        errs = []
+        for d in dirs:
+            fd = (loss(kappa + h * d) - loss(kappa - h * d)) / (2 * h)
+            ad = float(np.dot(g, d))
+            errs.append(abs(fd - ad) / (np.linalg.norm(g) * np.linalg.norm(d)))
+        report(f"      h={h:g}: max rel FD error {max(errs):.2e}")
+        assert max(errs) < tol
     s.close()
 
 
@@ -88,7 +91,8 @@ def test_torch_autograd_function(fem, gpu, oracle, meshgen):
     report("[gpu] torch.autograd.Function backward")
     xyz, tets, kappa, kappa2, b = _setup(meshgen, N=7, seed=11)
     s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
-    _tight(s, precision=1)
+    _tight(s, precision=0)
+    s.set_params(tolerance=1e-11)
     dev = torch.device("cuda:0")
     k_t = torch.tensor(kappa, dtype=torch.float64, device=dev, requires_grad=True)
     b_t = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
