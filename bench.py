@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-size", type=int, default=91, help="cube side of the CPU baseline sample")
     ap.add_argument("--skip", default="", help="comma list of: fp64,batched,adjoint,cpu")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
@@ -114,6 +116,14 @@ def main():
     skip = set(filter(None, a.skip.split(",")))
 
     import torch
+    if a.copy_probe:
+        src = torch.ones(a.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 5 copies of {a.copy_probe} MiB")
+        return
     fem = load_package()
     mg = fem.meshgen
     dev = torch.device("cuda:0")

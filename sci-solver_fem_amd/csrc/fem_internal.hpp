@@ -21,6 +21,9 @@
 
 namespace fem {
 
+constexpr int kSellWindows = 8;
+constexpr int kSellWinBits = 13;
+
 // Compressed sparse rows on the device.
 struct Csr {
   int64_t n = 0;    // rows
@@ -33,9 +36,18 @@ struct Csr {
   // Sliced ELL copy for the solve phase: slices of 64 consecutive rows (one
   // wavefront, a lane per row), stored column-major inside the slice and
   // padded to the longest row of the slice (padding: value 0, column = the row's last column).
+  // Columns are kept twice: as int32 (sc) and as 16-bit codes (sc16) for the
+  // compact slices, those whose columns fall in at most kSellWindows aligned
+  // windows of 2^kSellWinBits columns.  swin holds each slice's window bases
+  // (column offsets, multiples of 2^kSellWinBits; first entry -1 for a slice
+  // that is not compact); a code is (window << kSellWinBits) | (col - base).
   int64_t nslices = 0;
   DevBuf<int32_t> sp;   // nslices + 1 element offsets (multiples of 64)
   DevBuf<int32_t> sc;   // padded columns
+  DevBuf<uint16_t> sc16;  // padded 16-bit column codes (compact slices)
+  DevBuf<int32_t> swin;   // kSellWindows bases per slice
+  int64_t ncompact = 0;     // number of compact slices
+  int64_t nnz_compact = 0;  // stored nonzeros in compact slices
   DevBuf<double> s64;   // padded FP64 values
   DevBuf<float> s32;    // padded FP32 values (mixed precision only)
 };

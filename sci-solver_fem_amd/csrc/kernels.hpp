@@ -10,6 +10,8 @@
 
 #include <cstdint>
 
+#include "fem_internal.hpp"
+
 namespace fem {
 
 constexpr int kNT = 256;  // threads per block of the solve kernels
@@ -89,31 +91,84 @@ __global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const 
 // slice is column-major so every step k is one coalesced 64-wide load of
 // values and of columns, and the k steps are independent loads the compiler
 // keeps in flight together.  Sequential per-row sum (fixed order).
+struct SellView {
+  const int32_t* sp;    // nslices + 1 element offsets
+  const int32_t* sc;    // int32 columns
+  const uint16_t* sc16; // 16-bit column codes of compact slices
+  const int32_t* swin;  // kSellWindows window bases per slice (-1: not compact)
+};
+
+inline SellView sell_view(const Csr& A) { return SellView{A.sp.get(), A.sc.get(), A.sc16.get(), A.swin.get()}; }
+
+// Window bases of a compact slice, read once per wavefront into scalar
+// registers (named members, not an array: the select tree below must stay
+// register selects).
+struct SellWin {
+  int b0, b1, b2, b3, b4, b5, b6, b7;
+};
+
+__device__ __forceinline__ int sell_col(unsigned c, SellWin W) {
+  const bool s0 = c & (1u << kSellWinBits), s1 = c & (2u << kSellWinBits), s2 = c & (4u << kSellWinBits);
+  const int lo = s1 ? (s0 ? W.b3 : W.b2) : (s0 ? W.b1 : W.b0);
+  const int hi = s1 ? (s0 ? W.b7 : W.b6) : (s0 ? W.b5 : W.b4);
+  return (s2 ? hi : lo) + (int)(c & ((1u << kSellWinBits) - 1u));
+}
+
+// Row sum of slice `slice` (wave-uniform) for this lane's row.  Compact
+// slices read 2-byte column codes: bits 13-15 select one of the slice's 8
+// window bases, bits 0-12 are the offset.
+template <typename T>
+__device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane, const T* __restrict__ sv,
+                                          const T* __restrict__ x) {
+  const int off = __builtin_amdgcn_readfirstlane(S.sp[slice]);
+  const int w = __builtin_amdgcn_readfirstlane((S.sp[slice + 1] - off) >> 6);
+  const T* vp = sv + off + lane;
+  T acc = T(0);
+  static_assert(kSellWindows == 8, "sell_col selects among 8 windows");
+  const int32_t* wp = S.swin + slice * kSellWindows;
+  const SellWin W{__builtin_amdgcn_readfirstlane(wp[0]), __builtin_amdgcn_readfirstlane(wp[1]),
+                  __builtin_amdgcn_readfirstlane(wp[2]), __builtin_amdgcn_readfirstlane(wp[3]),
+                  __builtin_amdgcn_readfirstlane(wp[4]), __builtin_amdgcn_readfirstlane(wp[5]),
+                  __builtin_amdgcn_readfirstlane(wp[6]), __builtin_amdgcn_readfirstlane(wp[7])};
+  if (W.b0 >= 0) {
+    const uint16_t* cp = S.sc16 + off + lane;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+      const unsigned c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
+      const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+      acc += v0 * x[sell_col(c0, W)];
+      acc += v1 * x[sell_col(c1, W)];
+      acc += v2 * x[sell_col(c2, W)];
+      acc += v3 * x[sell_col(c3, W)];
+    }
+    for (; k < w; ++k) acc += vp[64 * k] * x[sell_col(cp[64 * k], W)];
+  } else {
+    const int32_t* cp = S.sc + off + lane;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+      const int c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
+      const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+      acc += v0 * x[c0];
+      acc += v1 * x[c1];
+      acc += v2 * x[c2];
+      acc += v3 * x[c3];
+    }
+    for (; k < w; ++k) acc += vp[64 * k] * x[cp[64 * k]];
+  }
+  return acc;
+}
+
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ sp,
-                                                 const int32_t* __restrict__ sc, const T* __restrict__ sv,
+__global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, SellView S, const T* __restrict__ sv,
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  const T* __restrict__ b, const T* __restrict__ dinv,
                                                  T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
-  const int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
+  // the slice index is the same on every lane of the wavefront
+  const int slice = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kNT / 64) + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
-  const int off = sp[slice];
-  const int w = (sp[slice + 1] - off) >> 6;
-  const int32_t* cp = sc + off + lane;
-  const T* vp = sv + off + lane;
-  T acc = T(0);
-  int k = 0;
-  for (; k + 4 <= w; k += 4) {
-    const int c0 = cp[64 * k], c1i = cp[64 * (k + 1)], c2i = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
-    const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
-    acc += v0 * x[c0];
-    acc += v1 * x[c1i];
-    acc += v2 * x[c2i];
-    acc += v3 * x[c3];
-  }
-  for (; k < w; ++k) acc += vp[64 * k] * x[cp[64 * k]];
-  const int64_t row = slice * 64 + lane;
+  const T acc = sell_row_sum<T>(S, slice, lane, sv, x);
+  const int64_t row = (int64_t)slice * 64 + lane;
   if (row < n) spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
 }
 

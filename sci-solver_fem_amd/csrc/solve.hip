@@ -109,8 +109,8 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
   const T* val = vals_of<T>(A);
   if (k == 1 && A.nslices > 0 && svals_of<T>(A)) {
     unsigned grid = grid_for(A.nslices, kNT / 64);
-    sell_kernel<T, MODE><<<grid, kNT, 0, s>>>(A.n, A.nslices, A.sp.get(), A.sc.get(), svals_of<T>(A), x, y, b, dinv,
-                                              d, xn, c1, c2);
+    sell_kernel<T, MODE><<<grid, kNT, 0, s>>>(A.n, A.nslices, sell_view(A), svals_of<T>(A), x, y, b, dinv, d, xn,
+                                              c1, c2);
   } else if (k == 1) {
     switch (lanes_for(A)) {
       case 4: launch_vec<T, T, MODE, 4>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
@@ -199,31 +199,16 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
 
 // q = A p on the sliced-ELL copy and per-block partials of p.q (one column);
 // blocks stride over groups of 4 slices so the partial count stays <= grid.
-__global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ sp,
-                                                         const int32_t* __restrict__ sc,
+__global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t nslices, SellView S,
                                                          const double* __restrict__ sv, const double* __restrict__ p,
                                                          double* __restrict__ q, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
   const int lane = threadIdx.x & 63;
   double part = 0.0;
-  for (int64_t slice = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6); slice < nslices;
-       slice += (int64_t)gridDim.x * (kNT / 64)) {
-    const int off = sp[slice];
-    const int w = (sp[slice + 1] - off) >> 6;
-    const int32_t* cp = sc + off + lane;
-    const double* vp = sv + off + lane;
-    double acc = 0.0;
-    int k = 0;
-    for (; k + 4 <= w; k += 4) {
-      const int c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
-      const double v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
-      acc += v0 * p[c0];
-      acc += v1 * p[c1];
-      acc += v2 * p[c2];
-      acc += v3 * p[c3];
-    }
-    for (; k < w; ++k) acc += vp[64 * k] * p[cp[64 * k]];
-    const int64_t row = slice * 64 + lane;
+  for (int slice = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kNT / 64) + (threadIdx.x >> 6)));
+       slice < nslices; slice += gridDim.x * (kNT / 64)) {
+    const double acc = sell_row_sum<double>(S, slice, lane, sv, p);
+    const int64_t row = (int64_t)slice * 64 + lane;
     if (row < n) {
       q[row] = acc;
       part += acc * p[row];
@@ -359,6 +344,16 @@ Smoother smoother_for(const Handle& h, const Level& L) {
   return S;
 }
 
+// Bytes of the stored matrix one SpMV with k columns reads: a value and a
+// column index per nonzero; the sliced-ELL copy (one column) stores 2-byte
+// column codes in its compact slices.  Padding is not counted.
+template <typename T>
+double matrix_bytes(const Csr& A, int k) {
+  const bool sell = k == 1 && A.nslices > 0;
+  const double nc = sell ? (double)A.nnz_compact : 0.0;
+  return (double)A.nnz * sizeof(T) + 2.0 * nc + 4.0 * ((double)A.nnz - nc);
+}
+
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
 // Buffers x and t ping-pong; returns the buffer that holds the result.
 template <typename T>
@@ -368,9 +363,9 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   const T* dinv = dinv_of<T>(L);
   Smoother S = smoother_for(h, L);
   const bool prof = h.profiling && l == 0;
-  // algorithmic bytes of one sweep: matrix (value + column per nonzero) and the
-  // per-row vectors (Jacobi: x, b, dinv, x_new; Chebyshev: + d read and write)
-  const double sweep_bytes = (double)A.nnz * (sizeof(T) + 4) + (double)A.n * sizeof(T) * (S.type ? 6.0 : 4.0) * k;
+  // algorithmic bytes of one sweep: the matrix and the per-row vectors
+  // (Jacobi: x, b, dinv, x_new; Chebyshev: + d read and write)
+  const double sweep_bytes = matrix_bytes<T>(A, k) + (double)A.n * sizeof(T) * (S.type ? 6.0 : 4.0) * k;
   if (nu <= 0) {
     if (zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(T) * A.n * k, s));
     return x;
@@ -441,8 +436,7 @@ T* vcycle(Handle& h, int l, const T* b, int k) {
   T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k);
   T* oth = (cur == x) ? t : x;
   {
-    ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", (double)A.nnz * (sizeof(T) + 4) +
-                                                                 3.0 * A.n * k * sizeof(T));
+    ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", matrix_bytes<T>(A, k) + 3.0 * A.n * k * sizeof(T));
     spmv_op<T, kResid>(A, k, cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   }
   Level& N = *H.levels[l + 1];
@@ -616,8 +610,8 @@ void update(Handle& h, int k, double tol2) {
 void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
   unsigned g = std::min<unsigned>(grid_for(A.nslices, kNT / 64), 2048u);
-  pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, A.sp.get(), A.sc.get(), A.s64.get(), h.kp.get(),
-                                               h.kq.get(), h.partials.get());
+  pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(),
+                                               h.partials.get());
   HIP_LAUNCH_CHECK();
   finish(h, (int)g, 1, F_ALPHA, 0);
 }
@@ -646,7 +640,7 @@ void launch_spmm_dot(Handle& h, int k) {
 
 void spmv_dot(Handle& h, int k) {
   const Csr& A = h.A;
-  ProfScope ps(h, "pcg_spmv", (double)A.nnz * 12 + 3.0 * 8 * A.n * k);
+  ProfScope ps(h, "pcg_spmv", matrix_bytes<double>(A, k) + 3.0 * 8 * A.n * k);
   if (k == 1 && A.nslices > 0 && A.s64.size()) {
     launch_sell_dot(h);
   } else if (k == 1) {

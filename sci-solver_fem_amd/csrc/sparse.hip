@@ -253,11 +253,51 @@ __global__ void sell_width_kernel(int64_t n, int64_t nslices, const int32_t* __r
   w[sl] = mx * 64;
 }
 
+// Window bases of each slice: the distinct aligned blocks (col >> kSellWinBits)
+// its columns touch, in order of first appearance (rows in order, entries in
+// row order).  More than kSellWindows blocks, or no entry at all: not compact.
+__global__ void sell_window_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+                                   const int32_t* __restrict__ ci, int32_t* __restrict__ swin,
+                                   int32_t* __restrict__ compact) {
+  int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslices) return;
+  int blk[kSellWindows];
+  int nb = 0;
+  bool ok = true;
+  for (int64_t r = sl * 64; r < sl * 64 + 64 && r < n && ok; ++r) {
+    for (int q = rp[r]; q < rp[r + 1]; ++q) {
+      const int b = ci[q] >> kSellWinBits;
+      bool seen = false;
+      for (int i = 0; i < nb; ++i) seen = seen || blk[i] == b;
+      if (seen) continue;
+      if (nb == kSellWindows) {
+        ok = false;
+        break;
+      }
+      blk[nb++] = b;
+    }
+  }
+  ok = ok && nb > 0;
+  for (int i = 0; i < kSellWindows; ++i)
+    swin[sl * kSellWindows + i] = !ok ? -1 : (i < nb ? (blk[i] << kSellWinBits) : (blk[0] << kSellWinBits));
+  compact[sl] = ok ? 1 : 0;
+}
+
+// stored nonzeros of each compact slice (0 for the others)
+__global__ void sell_compact_nnz_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+                                        const int32_t* __restrict__ compact, int64_t* __restrict__ out) {
+  int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslices) return;
+  const int64_t r0 = sl * 64, r1 = min(n, r0 + 64);
+  out[sl] = compact[sl] ? (int64_t)(rp[r1] - rp[r0]) : 0;
+}
+
 // one thread per (slice, lane): row = slice * 64 + lane
 __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
                                  const int32_t* __restrict__ ci, const double* __restrict__ v,
-                                 const int32_t* __restrict__ sp, int32_t* __restrict__ sc,
-                                 double* __restrict__ s64, float* __restrict__ s32) {
+                                 const int32_t* __restrict__ sp, const int32_t* __restrict__ swin,
+                                 int32_t* __restrict__ sc, uint16_t* __restrict__ sc16, double* __restrict__ s64,
+                                 float* __restrict__ s32) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslices * 64) return;
   int64_t sl = t >> 6;
@@ -270,21 +310,30 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
     base = rp[row];
     len = rp[row + 1] - base;
   }
+  const int32_t* win = swin + sl * kSellWindows;
+  const bool compact = win[0] >= 0;
   // padding repeats the row's last column (a valid index for rectangular P
-  // too, and already in cache); empty rows and rows past n use column 0
-  int pad_col = len > 0 ? ci[base + len - 1] : 0;
+  // too, and already in cache); empty rows and rows past n use the slice's
+  // first window base (a column at or below one the slice holds) or column 0
+  int pad_col = len > 0 ? ci[base + len - 1] : (compact ? win[0] : 0);
   for (int k = 0; k < w; ++k) {
     int64_t e = off + (int64_t)k * 64 + lane;
-    if (k < len) {
-      sc[e] = ci[base + k];
-      double val = v[base + k];
-      s64[e] = val;
-      if (s32) s32[e] = (float)val;
+    const int col = k < len ? ci[base + k] : pad_col;
+    sc[e] = col;
+    if (compact) {
+      int slot = 0;
+      for (int i = 0; i < kSellWindows; ++i)
+        if (win[i] == ((col >> kSellWinBits) << kSellWinBits)) {
+          slot = i;
+          break;
+        }
+      sc16[e] = (uint16_t)((slot << kSellWinBits) | (col - win[slot]));
     } else {
-      sc[e] = pad_col;
-      s64[e] = 0.0;
-      if (s32) s32[e] = 0.0f;
+      sc16[e] = 0;
     }
+    const double val = k < len ? v[base + k] : 0.0;
+    s64[e] = val;
+    if (s32) s32[e] = (float)val;
   }
 }
 
@@ -372,11 +421,24 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s) {
   int64_t tot = dev_exclusive_scan_i32(w.get(), A.sp.get(), ns, s);
   if (tot >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "sliced-ELL copy has 2^31 or more entries");
   A.sc.resize(tot > 0 ? tot : 1);
+  A.sc16.resize(tot > 0 ? tot : 1);
+  A.swin.resize(ns > 0 ? ns * kSellWindows : kSellWindows);
   A.s64.resize(tot > 0 ? tot : 1);
   if (with_f32) A.s32.resize(tot > 0 ? tot : 1); else A.s32.release();
+  A.ncompact = 0;
+  A.nnz_compact = 0;
   if (ns > 0) {
+    DevBuf<int32_t> cflag(ns), cscan(ns + 1);
+    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.swin.get(), cflag.get());
+    HIP_LAUNCH_CHECK();
+    A.ncompact = dev_exclusive_scan_i32(cflag.get(), cscan.get(), ns, s);
+    DevBuf<int64_t> snnz(ns), sscan(ns + 1);
+    sell_compact_nnz_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), cflag.get(), snnz.get());
+    HIP_LAUNCH_CHECK();
+    A.nnz_compact = dev_exclusive_scan_i64(snnz.get(), sscan.get(), ns, s);
     sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), A.sp.get(),
-                                                            A.sc.get(), A.s64.get(), with_f32 ? A.s32.get() : nullptr);
+                                                            A.swin.get(), A.sc.get(), A.sc16.get(), A.s64.get(),
+                                                            with_f32 ? A.s32.get() : nullptr);
     HIP_LAUNCH_CHECK();
   }
   HIP_CHECK(hipStreamSynchronize(s));
