@@ -31,33 +31,76 @@ STREAM_RHS = 4
 STREAM_NOISE = 5
 
 
+_CHUNK = 1 << 18  # elements per block: keeps the temporaries cache-resident
+
+
 def _mix(z: np.ndarray) -> np.ndarray:
-    z = (z ^ (z >> np.uint64(30))) * _M1
-    z = (z ^ (z >> np.uint64(27))) * _M2
-    return z ^ (z >> np.uint64(31))
+    """SplitMix64 finaliser, in place on z (uint64)."""
+    t = np.empty_like(z)
+    np.right_shift(z, np.uint64(30), out=t)
+    z ^= t
+    z *= _M1
+    np.right_shift(z, np.uint64(27), out=t)
+    z ^= t
+    z *= _M2
+    np.right_shift(z, np.uint64(31), out=t)
+    z ^= t
+    return z
+
+
+def _stream_key(seed: int, stream: int) -> np.uint64:
+    with np.errstate(over="ignore"):
+        return _mix(np.array([(seed * 0x100000001B3 + stream * 0x632BE59BD9B4E019)
+                              & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64))[0]
+
+
+def _words(key: np.uint64, lo: int, hi: int) -> np.ndarray:
+    """Words with counters lo+1 .. hi of the stream with this key."""
+    with np.errstate(over="ignore"):
+        z = np.arange(lo + 1, hi + 1, dtype=np.uint64)
+        z *= _GOLDEN
+        z += key
+        return _mix(z)
 
 
 def splitmix64(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
     """n 64-bit words of the counter-based stream (seed, stream)."""
-    with np.errstate(over="ignore"):
-        key = _mix(np.array([(seed * 0x100000001B3 + stream * 0x632BE59BD9B4E019)
-                             & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64))[0]
-        ctr = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
-        return _mix(key + ctr * _GOLDEN)
+    key = _stream_key(seed, stream)
+    out = np.empty(n, dtype=np.uint64)
+    for a in range(0, n, _CHUNK):
+        b = min(n, a + _CHUNK)
+        out[a:b] = _words(key, offset + a, offset + b)
+    return out
+
+
+def _unit(w: np.ndarray) -> np.ndarray:
+    w >>= np.uint64(11)
+    u = w.astype(np.float64)
+    u *= 1.0 / 9007199254740992.0
+    return u
 
 
 def uniform(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
     """U[0,1) doubles with 53 random bits."""
-    w = splitmix64(seed, stream, n, offset)
-    return (w >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    key = _stream_key(seed, stream)
+    out = np.empty(n, dtype=np.float64)
+    for a in range(0, n, _CHUNK):
+        b = min(n, a + _CHUNK)
+        out[a:b] = _unit(_words(key, offset + a, offset + b))
+    return out
 
 
 def normal(seed: int, stream: int, n: int) -> np.ndarray:
-    """N(0,1) by Box-Muller on pairs of uniforms (cosine branch only)."""
-    u1 = uniform(seed, stream, n, 0)
-    u2 = uniform(seed, stream, n, n)
-    u1 = 1.0 - u1  # (0, 1]
-    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    """N(0,1) by Box-Muller on pairs of uniforms (cosine branch only):
+    g_i = sqrt(-2 ln(1 - u_i)) cos(2 pi u_{n+i}), u = uniform(seed, stream)."""
+    key = _stream_key(seed, stream)
+    out = np.empty(n, dtype=np.float64)
+    for a in range(0, n, _CHUNK):
+        b = min(n, a + _CHUNK)
+        u1 = 1.0 - _unit(_words(key, a, b))  # (0, 1]
+        u2 = _unit(_words(key, n + a, n + b))
+        out[a:b] = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    return out
 
 
 def random_permutation(seed: int, stream: int, n: int) -> np.ndarray:
@@ -82,51 +125,71 @@ def _kuhn_local():
     return np.array(tets, dtype=np.int64)
 
 
-def _orient_tets(xyz: np.ndarray, tets: np.ndarray) -> np.ndarray:
-    p0 = xyz[tets[:, 0]]
-    d1 = xyz[tets[:, 1]] - p0
-    d2 = xyz[tets[:, 2]] - p0
-    d3 = xyz[tets[:, 3]] - p0
-    det = np.einsum("ij,ij->i", d1, np.cross(d2, d3))
-    neg = det < 0
-    t = tets.copy()
-    t[neg, 1], t[neg, 2] = tets[neg, 2], tets[neg, 1]
-    return t, np.abs(det)
+def _local_kuhn_oriented():
+    """The 6 Kuhn tets of the unit cube with corners 1 and 2 swapped where the
+    corner order is negatively oriented, so every tet has positive volume."""
+    local = _kuhn_local()
+    corner = np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)], dtype=np.float64)
+    out = local.copy()
+    for t in range(local.shape[0]):
+        p = corner[local[t]]
+        if np.linalg.det(p[1:] - p[0]) < 0:
+            out[t, 1], out[t, 2] = local[t, 2], local[t, 1]
+    return out
 
 
 def cube_mesh(N: int, seed: int = 0, jitter: float = 0.2, permute: bool = True):
     """Unit-cube Kuhn tet mesh on an N^3 vertex grid.
 
     Returns (xyz float64 [nv,3], tets int32 [ne,4]) with nv = N^3 and
-    ne = 6 (N-1)^3.
+    ne = 6 (N-1)^3.  Grid vertex (i,j,k) is i*N*N + j*N + k before the
+    permutation; tets are emitted cell by cell (cells in the same order), 6
+    per cell.  Every tet is checked to have positive volume after the jitter.
     """
     h = 1.0 / (N - 1)
+    nv = N * N * N
     g = np.arange(N, dtype=np.float64) * h
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")  # vertex (i,j,k) -> i*N*N + j*N + k
-    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
-    ijk = np.stack(np.meshgrid(*(np.arange(N),) * 3, indexing="ij"), -1).reshape(-1, 3)
-    interior = np.all((ijk > 0) & (ijk < N - 1), axis=1)
+    X = np.repeat(g, N * N)
+    Y = np.tile(np.repeat(g, N), N)
+    Z = np.tile(g, N * N)
     if jitter:
-        u = uniform(seed, STREAM_JITTER, 3 * xyz.shape[0]).reshape(-1, 3)
-        xyz[interior] += ((2.0 * u - 1.0) * (jitter * h))[interior]
-    # cells
-    c = np.arange(N - 1)
-    ci, cj, ck = np.meshgrid(c, c, c, indexing="ij")
-    base = (ci * N * N + cj * N + ck).ravel()
+        inner = np.zeros(N, dtype=bool)
+        inner[1:N - 1] = True
+        interior = (inner[:, None, None] & inner[None, :, None] & inner[None, None, :]).ravel()
+        u = uniform(seed, STREAM_JITTER, 3 * nv).reshape(-1, 3)
+        for d, C in enumerate((X, Y, Z)):
+            C[interior] += ((2.0 * u[interior, d] - 1.0) * (jitter * h))
+        del u, interior
+    if permute:
+        perm = random_permutation(seed, STREAM_PERM, nv)  # new id -> old id
+        inv = np.empty(nv, dtype=np.int32)
+        inv[perm] = np.arange(nv, dtype=np.int32)
     corner_off = np.array([dx * N * N + dy * N + dz
                            for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)], dtype=np.int64)
-    local = _kuhn_local()  # (6,4) corner ids
-    tets = (base[:, None, None] + corner_off[local][None, :, :]).reshape(-1, 4)
-    tets, vol6 = _orient_tets(xyz, tets)
-    if not np.all(vol6 > 0):
-        raise ValueError("generated mesh has a degenerate or inverted tetrahedron")
+    local = corner_off[_local_kuhn_oriented()]  # (6, 4) vertex offsets from the cell's base
+    nc1 = N - 1
+    ncell = nc1 ** 3
+    tets = np.empty((6 * ncell, 4), dtype=np.int32)
+    step = max(1, _CHUNK // 6)
+    for a in range(0, ncell, step):
+        b = min(ncell, a + step)
+        c = np.arange(a, b, dtype=np.int64)
+        ci, rem = np.divmod(c, nc1 * nc1)
+        cj, ck = np.divmod(rem, nc1)
+        base = ci * N * N + cj * N + ck
+        t = (base[:, None, None] + local[None, :, :]).reshape(-1, 4)
+        c0, c1, c2, c3 = (t[:, i] for i in range(4))
+        ax, ay, az = X[c1] - X[c0], Y[c1] - Y[c0], Z[c1] - Z[c0]
+        bx, by, bz = X[c2] - X[c0], Y[c2] - Y[c0], Z[c2] - Z[c0]
+        cx, cy, cz = X[c3] - X[c0], Y[c3] - Y[c0], Z[c3] - Z[c0]
+        det = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx)
+        if not np.all(det > 0):
+            raise ValueError("generated mesh has a degenerate or inverted tetrahedron")
+        tets[6 * a:6 * b] = inv[t] if permute else t
+    xyz = np.stack([X, Y, Z], axis=1)
     if permute:
-        perm = random_permutation(seed, STREAM_PERM, xyz.shape[0])  # new id -> old id
-        inv = np.empty_like(perm)
-        inv[perm] = np.arange(perm.size)
         xyz = xyz[perm]
-        tets = inv[tets]
-    return np.ascontiguousarray(xyz), np.ascontiguousarray(tets.astype(np.int32))
+    return np.ascontiguousarray(xyz), tets
 
 
 def square_mesh(N: int, seed: int = 0, jitter: float = 0.2, permute: bool = True):
