@@ -96,7 +96,54 @@ def cpu_baseline(fem, size: int):
             "converged": res["converged"], "rel_residual": res["rel_residual"]}
 
 
+DOMINANT_KERNEL = "sell_kernel<float, 4>"  # level-0 Chebyshev step (FP32 sliced ELL, fused epilogue)
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_level0_chebyshev_step.json"
+
+
+def _pmc_rows(path, counter):
+    """Per-dispatch (kernel name, grid size, value) of one rocprofv3 --pmc CSV."""
+    import csv
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") == counter:
+                out.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return out
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib, command):
+    """HBM bytes per launch of the dominant kernel from two --pmc runs (FETCH_SIZE,
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for
+    a plain device copy of copy_mib MiB (bench.py --copy-probe)."""
+    def avg_of(rows, pred):
+        v = [x for (k, g, x) in rows if pred(k, g)]
+        return sum(v) / len(v) if v else None
+
+    known_kib = copy_mib * 1024.0
+    is_copy = lambda k, g: "copyBuffer" in k
+    f_fac = known_kib / avg_of(_pmc_rows(copy_fetch_csv, "FETCH_SIZE"), is_copy)
+    w_fac = known_kib / avg_of(_pmc_rows(copy_write_csv, "WRITE_SIZE"), is_copy)
+    fr = _pmc_rows(fetch_csv, "FETCH_SIZE")
+    wr = _pmc_rows(write_csv, "WRITE_SIZE")
+    grid = max(g for (k, g, x) in fr if DOMINANT_KERNEL in k)  # the finest level
+    pick = lambda k, g: DOMINANT_KERNEL in k and g == grid
+    fetch = avg_of(fr, pick) * 1024.0 * f_fac
+    write = avg_of(wr, pick) * 1024.0 * w_fac
+    res = {"kernel": DOMINANT_KERNEL, "grid_threads": grid, "fetch_bytes": round(fetch), "write_bytes": round(write),
+           "traffic_bytes": round(fetch + write), "fetch_scale": round(f_fac, 4), "write_scale": round(w_fac, 4),
+           "dispatches": sum(1 for (k, g, x) in fr if pick(k, g)), "command": command}
+    return res
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
+        # bench.py --pmc-summary FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_MIB ALGO_BYTES "command"
+        f, w, cf, cw, mib, algo, cmd = sys.argv[2:9]
+        res = pmc_summary(f, w, cf, cw, float(mib), cmd)
+        res["algorithmic_bytes"] = int(algo)
+        TRAFFIC_FILE.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res, indent=1))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -206,8 +253,15 @@ def main():
     if prof and prof["launches"]:
         avg_ms = prof["total_ms"] / prof["launches"]
         ach = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        # HBM bytes per launch measured by rocprofv3 counters (committed under
+        # profiles/); used only when it was measured on this same workload
+        traffic = None
+        if TRAFFIC_FILE.exists():
+            tj = json.loads(TRAFFIC_FILE.read_text())
+            if int(tj.get("algorithmic_bytes", -1)) == int(prof["bytes_per_launch"]):
+                traffic = tj["traffic_bytes"]
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "level-0 Chebyshev smoother step (FP32 sliced-ELL SpMV fused with residual and update)",
                 "launches": prof["launches"], "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": prof["bytes_per_launch"]}

@@ -178,6 +178,7 @@ class FEMSolver {
   int aggregationRule_;
   unsigned long long seed_;
   double strengthThreshold_;
+  int launchMode_;
 
  private:
   void load_fields(const fem_params& p) {
@@ -205,6 +206,7 @@ class FEMSolver {
     aggregationRule_ = p.aggregationRule;
     seed_ = p.seed;
     strengthThreshold_ = p.strengthThreshold;
+    launchMode_ = p.launchMode;
   }
   void push_fields() {
     fem_params p;
@@ -233,6 +235,7 @@ class FEMSolver {
     p.aggregationRule = aggregationRule_;
     p.seed = seed_;
     p.strengthThreshold = strengthThreshold_;
+    p.launchMode = launchMode_;
     if (fem_set_params(h_, &p) != FEM_OK) throw std::invalid_argument(std::string("FEMSolver: ") + fem_last_error(h_));
   }
 

@@ -78,6 +78,9 @@ typedef struct fem_params {
                              induced aggregate graph below (the reference);
                              > 0: aggregate the strength graph of every level,
                              |a_ij| >= theta sqrt(a_ii a_jj)                   */
+  int launchMode;         /* 1: each PCG iteration (V-cycle, dots, updates) is
+                             captured once as a HIP graph and replayed;
+                             0: kernels launched one by one (same results)    */
 } fem_params;
 
 /* Fill *p with the reference defaults (src/FEMSolver.cu:11-33). */

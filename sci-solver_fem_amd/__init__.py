@@ -69,6 +69,7 @@ _FIELDS = {
     # additions
     "precision_": "precision", "smootherType_": "smootherType",
     "aggregationRule_": "aggregationRule", "seed_": "seed", "strengthThreshold_": "strengthThreshold",
+    "launchMode_": "launchMode",
 }
 
 

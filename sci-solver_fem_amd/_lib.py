@@ -49,6 +49,7 @@ class FemParams(C.Structure):
         ("coarseSolve", C.c_int),
         ("seed", C.c_ulonglong),
         ("strengthThreshold", C.c_double),
+        ("launchMode", C.c_int),
     ]
 
 

@@ -618,6 +618,7 @@ void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int 
 
 void build_hierarchy(Handle& h) {
   auto t0 = std::chrono::steady_clock::now();
+  ++h.gen;  // invalidates a captured PCG iteration
   hipStream_t s = h.stream;
   Hierarchy& H = h.H;
   H.levels.clear();

@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kRowBlock = 64;   // threads (rows) per block for row-owner kernels
 constexpr int kRowCap = 64;     // LDS row capacity; longer rows use the global path
+constexpr int kAsmCap = 32;     // LDS row capacity of the value assembly (12 B per slot)
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
   v &= 0x1fffffull;
@@ -286,11 +287,20 @@ __device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool
     element_geometry(X, c, dim, G);
     double kv = kappa[e] * G.meas;
     double mv = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * mden;
-    int v[4] = {c.x, c.y, c.z, c.w};
-    for (int b = 0; b < npe; ++b) {
-      double s = G.g[a][0] * G.g[b][0] + G.g[a][1] * G.g[b][1] + G.g[a][2] * G.g[b][2];
-      double val = kv * s + mv * (a == b ? 2.0 : 1.0);
-      acc[find_col(cols, len, v[b])] += val;
+    const int v[4] = {c.x, c.y, c.z, c.w};
+    // gradient of corner a picked by selects, and the corner loop unrolled,
+    // so G and v stay in registers (a dynamic index would put them in scratch)
+    double ga[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      ga[d] = a == 0 ? G.g[0][d] : (a == 1 ? G.g[1][d] : (a == 2 ? G.g[2][d] : G.g[3][d]));
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b < npe) {
+        double s = ga[0] * G.g[b][0] + ga[1] * G.g[b][1] + ga[2] * G.g[b][2];
+        double val = kv * s + mv * (a == b ? 2.0 : 1.0);
+        acc[find_col(cols, len, v[b])] += val;
+      }
     }
   }
 }
@@ -300,15 +310,15 @@ __global__ void __launch_bounds__(kRowBlock) assemble_rows_kernel(
     const int32_t* __restrict__ v2e, const int4* __restrict__ conn, const double4* __restrict__ X,
     const double* __restrict__ kappa, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
     double* __restrict__ val) {
-  __shared__ int s_cols[kRowBlock * kRowCap];
-  __shared__ double s_acc[kRowBlock * kRowCap];
+  __shared__ int s_cols[kRowBlock * kAsmCap];
+  __shared__ double s_acc[kRowBlock * kAsmCap];
   int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nv) return;
   int o = rp[row];
   int len = rp[row + 1] - o;
-  if (len <= kRowCap) {
-    int* cols = s_cols + threadIdx.x * kRowCap;
-    double* acc = s_acc + threadIdx.x * kRowCap;
+  if (len <= kAsmCap) {
+    int* cols = s_cols + threadIdx.x * kAsmCap;
+    double* acc = s_acc + threadIdx.x * kAsmCap;
     for (int t = 0; t < len; ++t) { cols[t] = ci[o + t]; acc[t] = 0.0; }
     assemble_row(row, dim, npe, signed_mass != 0, mass_coeff, v2e_ptr, v2e, conn, X, kappa, cols, len, acc);
     for (int t = 0; t < len; ++t) val[o + t] = acc[t];
@@ -329,13 +339,17 @@ __global__ void element_gradient_kernel(int64_t ne, int dim, int npe, const int4
   int4 c = conn[e];
   Geo G;
   element_geometry(X, c, dim, G);
-  int v[4] = {c.x, c.y, c.z, c.w};
+  const int v[4] = {c.x, c.y, c.z, c.w};
   double ga[3] = {0, 0, 0}, gu[3] = {0, 0, 0};
-  for (int k = 0; k < npe; ++k) {
-    double av = a[v[k]], uv = u[v[k]];
-    for (int d = 0; d < 3; ++d) {
-      ga[d] += av * G.g[k][d];
-      gu[d] += uv * G.g[k][d];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < npe) {
+      double av = a[v[k]], uv = u[v[k]];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ga[d] += av * G.g[k][d];
+        gu[d] += uv * G.g[k][d];
+      }
     }
   }
   out[eperm[e]] = -G.meas * (ga[0] * gu[0] + ga[1] * gu[1] + ga[2] * gu[2]);
@@ -363,9 +377,64 @@ __global__ void scatter_kernel(int64_t n, int k, const int32_t* __restrict__ per
   out[(int64_t)perm[i] * k + c] = in[t];
 }
 
+// bit 0: a vertex index outside [0, nv); bit 1: NaN coefficient; bit 2: NaN coordinate
+__global__ void mesh_check_kernel(int64_t nconn, const int32_t* __restrict__ conn, int64_t nv, int64_t ne,
+                                  const double* __restrict__ kappa, int64_t nx, const double* __restrict__ xyz,
+                                  int* __restrict__ flags) {
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  int f = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nconn; t += st) {
+    const int c = conn[t];
+    if (c < 0 || c >= nv) f |= 1;
+  }
+  if (kappa)
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ne; t += st)
+      if (!(kappa[t] == kappa[t])) f |= 2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nx; t += st)
+    if (!(xyz[t] == xyz[t])) f |= 4;
+  if (f) atomicOr(flags, f);
+}
+
+// per-block (min x, min y, min z, max x, max y, max z) of the vertex coordinates
+__global__ void __launch_bounds__(256) bbox_kernel(int64_t nv, const double* __restrict__ xyz,
+                                                   double* __restrict__ part) {
+  __shared__ double s_v[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x)
+    for (int d = 0; d < 3; ++d) {
+      const double c = xyz[3 * i + d];
+      v[d] = fmin(v[d], c);
+      v[3 + d] = fmax(v[3 + d], c);
+    }
+  for (int j = 0; j < 6; ++j) s_v[j][threadIdx.x] = v[j];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int j = 0; j < 6; ++j)
+        s_v[j][threadIdx.x] = j < 3 ? fmin(s_v[j][threadIdx.x], s_v[j][threadIdx.x + w])
+                                    : fmax(s_v[j][threadIdx.x], s_v[j][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s_v[threadIdx.x][0];
+}
+
 }  // namespace
 
-void build_mesh_device(Handle& h) {
+int check_mesh_device(int64_t nconn, const int32_t* conn, int64_t nv, int64_t ne, const double* kappa,
+                      int64_t nx, const double* xyz, hipStream_t s) {
+  DevBuf<int> flags(1);
+  dev_fill_i32(flags.get(), 0, 1, s);
+  const int64_t m = std::max(nconn, std::max(ne, nx));
+  mesh_check_kernel<<<(unsigned)std::min<int64_t>(std::max<int64_t>((m + 255) / 256, 1), 4096), 256, 0, s>>>(
+      nconn, conn, nv, ne, kappa, nx, xyz, flags.get());
+  HIP_LAUNCH_CHECK();
+  int f = 0;
+  HIP_CHECK(hipMemcpyAsync(&f, flags.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return f;
+}
+
+void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const double* kappa) {
   hipStream_t s = h.stream;
   const int64_t nv = h.nv, ne = h.ne;
   const int npe = h.dim + 1;
@@ -373,22 +442,37 @@ void build_mesh_device(Handle& h) {
   if (4 * ne >= (int64_t)1 << 31 || nv >= (int64_t)1 << 31)
     fail(FEM_ERR_CAPACITY, "mesh too large for 32-bit element/vertex indices");
 
-  // bounding box on the host copy
+  DevBuf<double> xyz_u;
+  xyz_u.upload(xyz, (size_t)(3 * nv), s);
+  DevBuf<int32_t> conn_u;
+  conn_u.upload(conn, (size_t)(npe * ne), s);
+  DevBuf<double> kappa_u(ne);
+  if (kappa) kappa_u.upload(kappa, (size_t)ne, s);
+  else dev_fill_f64(kappa_u.get(), 1.0, (size_t)ne, s);
+  const int bad = check_mesh_device(npe * ne, conn_u.get(), nv, ne, kappa ? kappa_u.get() : nullptr, 3 * nv,
+                                    xyz_u.get(), s);
+  if (bad & 1) fail(FEM_ERR_ARG, "element vertex index out of range");
+  if (bad & 2) fail(FEM_ERR_NUMERIC, "NaN in element coefficient");
+  if (bad & 4) fail(FEM_ERR_NUMERIC, "NaN in vertex coordinates");
+
+  // bounding box: per-block min/max on the device, the last fold on the host
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int64_t v = 0; v < nv; ++v)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = std::min(lo[d], h.h_xyz[3 * v + d]);
-      hi[d] = std::max(hi[d], h.h_xyz[3 * v + d]);
-    }
+  {
+    const unsigned nb = (unsigned)std::min<int64_t>((nv + 255) / 256, 1024);
+    DevBuf<double> part(6 * (size_t)nb);
+    bbox_kernel<<<nb, 256, 0, s>>>(nv, xyz_u.get(), part.get());
+    HIP_LAUNCH_CHECK();
+    std::vector<double> hp(6 * (size_t)nb);
+    part.download(hp.data(), hp.size(), s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (unsigned b = 0; b < nb; ++b)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], hp[6 * b + d]);
+        hi[d] = std::max(hi[d], hp[6 * b + 3 + d]);
+      }
+  }
   double sc[3];
   for (int d = 0; d < 3; ++d) sc[d] = (hi[d] > lo[d]) ? 1.0 / (hi[d] - lo[d]) : 0.0;
-
-  DevBuf<double> xyz_u;
-  xyz_u.upload(h.h_xyz.data(), (size_t)(3 * nv), s);
-  DevBuf<int32_t> conn_u;
-  conn_u.upload(h.h_conn.data(), (size_t)(npe * ne), s);
-  DevBuf<double> kappa_u;
-  kappa_u.upload(h.h_kappa.data(), (size_t)ne, s);
 
   // vertex order
   {

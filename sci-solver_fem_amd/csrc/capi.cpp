@@ -64,25 +64,13 @@ void create_common(Handle& h, int dim, int64_t nv, const double* xyz, int64_t ne
                    const double* kappa, double mass, int flags) {
   if (dim != 2 && dim != 3) fail(FEM_ERR_ARG, "dim must be 2 or 3");
   if (nv <= 0 || ne <= 0 || !xyz || !conn) fail(FEM_ERR_ARG, "empty mesh");
-  const int npe = dim + 1;
-  for (int64_t t = 0; t < ne * npe; ++t)
-    if (conn[t] < 0 || conn[t] >= nv) fail(FEM_ERR_ARG, "element vertex index out of range");
-  if (kappa)
-    for (int64_t e = 0; e < ne; ++e)
-      if (!(kappa[e] == kappa[e])) fail(FEM_ERR_NUMERIC, "NaN in element coefficient");
-  for (int64_t t = 0; t < 3 * nv; ++t)
-    if (!(xyz[t] == xyz[t])) fail(FEM_ERR_NUMERIC, "NaN in vertex coordinates");
   h.dim = dim;
   h.nv = nv;
   h.ne = ne;
-  h.h_xyz.assign(xyz, xyz + 3 * nv);
-  h.h_conn.assign(conn, conn + npe * ne);
-  h.h_kappa.assign(ne, 1.0);
-  if (kappa) std::copy(kappa, kappa + ne, h.h_kappa.begin());
   h.mass_coeff = mass;
   h.flags = flags;
   init_device(h);
-  build_mesh_device(h);
+  build_mesh_device(h, xyz, conn, kappa);  // validates indices and NaNs on the device
   HIP_CHECK(hipStreamSynchronize(h.stream));
 }
 
@@ -233,6 +221,7 @@ void fem_params_default(fem_params* p) {
   p->coarseSolve = 0;
   p->seed = 0x5EEDull;
   p->strengthThreshold = 0.0;
+  p->launchMode = 1;
 }
 
 int fem_create_from_arrays(int dim, int64_t nv, const double* xyz, int64_t ne, const int32_t* conn,
@@ -277,6 +266,7 @@ void fem_destroy(fem_handle* hh) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->h_scal) (void)hipHostFree(h->h_scal);
+  if (h->iter_graph) (void)hipGraphExecDestroy(h->iter_graph);
   release_profile_events(*h);
   hipStream_t s = h->own_stream ? h->stream : nullptr;
   delete h;  // device buffers free themselves
@@ -295,12 +285,13 @@ int fem_set_params(fem_handle* hh, const fem_params* p) {
     if (p->maxIters < 0 || p->topSize < 1 || p->maxLevels < 1 || p->preInnerIters < 0 || p->postInnerIters < 0 ||
         !(p->tolerance >= 0) || p->precision < 0 || p->precision > 1 || p->smootherType < 0 ||
         p->smootherType > 1 || p->aggregationRule < 0 || p->aggregationRule > 1 || !(p->strengthThreshold >= 0) ||
-        p->strengthThreshold >= 1.0)
+        p->strengthThreshold >= 1.0 || p->launchMode < 0 || p->launchMode > 1)
       fail(FEM_ERR_ARG, "invalid solver parameter");
     if (p->aggregatorType != 0) fail(FEM_ERR_ARG, "only aggregatorType 0 (randomized MIS) is implemented");
     if (params_shape_changed(h.prm, *p)) h.H.built = false;
     if (p->device != h.prm.device) fail(FEM_ERR_ARG, "device cannot change after construction");
     h.prm = *p;
+    ++h.gen;  // smoother coefficients and sweep counts are baked into a captured iteration
     return (int)FEM_OK;
   });
 }
@@ -406,11 +397,10 @@ int fem_update_kappa(fem_handle* hh, const double* kappa) {
   return guarded(hh, [&]() {
     Handle& h = *H(hh);
     if (!h.has_mesh || h.matrix_from_file) fail(FEM_ERR_STATE, "operator is not assembled from a mesh");
-    for (int64_t e = 0; e < h.ne; ++e)
-      if (!(kappa[e] == kappa[e])) fail(FEM_ERR_NUMERIC, "NaN in element coefficient");
-    std::copy(kappa, kappa + h.ne, h.h_kappa.begin());
     DevBuf<double> ku;
     ku.upload(kappa, h.ne, h.stream);
+    if (check_mesh_device(0, nullptr, 0, h.ne, ku.get(), 0, nullptr, h.stream) & 2)
+      fail(FEM_ERR_NUMERIC, "NaN in element coefficient");
     set_kappa_user(h, ku.get());
     assemble_values(h);
     HIP_CHECK(hipStreamSynchronize(h.stream));

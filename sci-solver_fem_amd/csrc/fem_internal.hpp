@@ -98,12 +98,9 @@ struct Handle {
   bool own_stream = false;
   int device = 0;
 
-  // mesh (host copies, user numbering; kept for IO and neighbour queries)
+  // mesh sizes (the arrays themselves live on the device only)
   int dim = 3;
   int64_t nv = 0, ne = 0;
-  std::vector<double> h_xyz;     // nv x 3
-  std::vector<int32_t> h_conn;   // ne x (dim+1)
-  std::vector<double> h_kappa;   // ne
   double mass_coeff = 1.0;
   int flags = 0;
 
@@ -134,6 +131,13 @@ struct Handle {
   double* h_scal = nullptr;       // pinned mirror of scal
   DevBuf<double> ubuf, ubuf2;     // user-order staging (nv * k)
 
+  // captured PCG iteration (launchMode 1), valid while gen == graph_gen
+  uint64_t gen = 1;             // bumped when buffers, hierarchy or parameters change
+  hipGraphExec_t iter_graph = nullptr;
+  uint64_t graph_gen = 0;
+  int graph_k = 0;
+  double graph_tol2 = 0.0;
+
   bool profiling = false;
   std::map<std::string, KernelStat> stats;
   std::vector<PendingTiming> pending;   // recorded, not yet read back
@@ -155,7 +159,12 @@ void dev_iota_i32(int32_t* p, size_t n, hipStream_t s);
 void dev_f64_to_f32(const double* in, float* out, size_t n, hipStream_t s);
 
 // ---- mesh, pattern and assembly (assembly.hip) ----
-void build_mesh_device(Handle& h);          // reorder + v2e + pattern + values
+// reorder + v2e + pattern + values from the caller's host arrays (user
+// numbering; kappa may be null = 1); validates them on the device
+void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const double* kappa);
+// bit 0: vertex index out of range, bit 1: NaN coefficient, bit 2: NaN coordinate
+int check_mesh_device(int64_t nconn, const int32_t* conn, int64_t nv, int64_t ne, const double* kappa, int64_t nx,
+                      const double* xyz, hipStream_t s);
 void assemble_values(Handle& h);            // values only, pattern fixed
 void element_gradient(Handle& h, const double* a_int, const double* u_int, double* g_user);
 void gather_user_to_int(Handle& h, const double* u, double* i, int k);   // i[r] = u[perm[r]]

@@ -24,6 +24,26 @@ enum SpmvMode : int {
   kCheb = 4,     // r = dinv (b - A x); d = c1 d + c2 r; xn = x + d
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Gathering kernels remap the hardware block index so that XCD j runs a
+// contiguous range of logical blocks: neighbouring rows (Morton order) then
+// share one L2 and the gathered vector is fetched from HBM about once
+// instead of once per XCD.  A bijection of [0, nblocks) for any nblocks.
+constexpr unsigned kNumXcd = 8;
+
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
+  const unsigned x = bid % kNumXcd, i = bid / kNumXcd;
+  const unsigned q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+  return x * q + min(x, r) + i;
+}
+
+// Matrix entries are read once per SpMV: nontemporal loads keep them from
+// evicting the gathered vector from L2 (measured: -15 % time at 4.2M rows).
+template <typename T>
+__device__ __forceinline__ T ntload(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_xor_t(T v, int mask, int width) {
   return __shfl_xor(v, mask, width);
@@ -43,9 +63,9 @@ __device__ __forceinline__ void spmv_epilogue(int64_t idx, int64_t row, T acc, T
     y[idx] += acc;
   } else if (MODE == kJacobi) {
     xn[idx] = x[idx] + c1 * (T)dinv[row] * (b[idx] - acc);
-  } else {  // kCheb
-    T r = (T)dinv[row] * (b[idx] - acc);
-    T dn = (c1 != T(0) ? c1 * d[idx] : T(0)) + c2 * r;
+  } else {  // kCheb (b, D^-1 and d are streamed once: nontemporal)
+    T r = (T)ntload(dinv + row) * (ntload(b + idx) - acc);
+    T dn = (c1 != T(0) ? c1 * ntload(d + idx) : T(0)) + c2 * r;
     d[idx] = dn;
     xn[idx] = x[idx] + dn;
   }
@@ -59,12 +79,12 @@ __global__ void __launch_bounds__(kNT) csr_vec_kernel(int64_t n, const int32_t* 
                                                     const T* __restrict__ x, T* __restrict__ y,
                                                     const T* __restrict__ b, const V* __restrict__ dinv,
                                                     T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
-  const int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / G;
+  const int64_t row = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kNT + threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1);
   if (row >= n) return;
   const int s = rp[row], e = rp[row + 1];
   T acc = T(0);
-  for (int k = s + lane; k < e; k += G) acc += (T)val[k] * x[ci[k]];
+  for (int k = s + lane; k < e; k += G) acc += (T)ntload(val + k) * x[ntload(ci + k)];
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) acc += shfl_xor_t(acc, off, G);
   if (lane == 0) spmv_epilogue<MODE, T, V>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
@@ -78,12 +98,12 @@ __global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const 
                                                       const T* __restrict__ x, T* __restrict__ y,
                                                       const T* __restrict__ b, const V* __restrict__ dinv,
                                                       T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
-  const int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP;
+  const int64_t row = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kNT + threadIdx.x) / KP;
   const int c = threadIdx.x & (KP - 1);
   if (row >= n || c >= k) return;
   const int s = rp[row], e = rp[row + 1];
   T acc = T(0);
-  for (int q = s; q < e; ++q) acc += (T)val[q] * x[(int64_t)ci[q] * k + c];
+  for (int q = s; q < e; ++q) acc += (T)ntload(val + q) * x[(int64_t)ntload(ci + q) * k + c];
   spmv_epilogue<MODE, T, V>(row * k + c, row, acc, y, x, b, dinv, d, xn, c1, c2);
 }
 
@@ -134,26 +154,30 @@ __device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane
     const uint16_t* cp = S.sc16 + off + lane;
     int k = 0;
     for (; k + 4 <= w; k += 4) {
-      const unsigned c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
-      const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+      const unsigned c0 = ntload(cp + 64 * k), c1 = ntload(cp + 64 * (k + 1)), c2 = ntload(cp + 64 * (k + 2)),
+                     c3 = ntload(cp + 64 * (k + 3));
+      const T v0 = ntload(vp + 64 * k), v1 = ntload(vp + 64 * (k + 1)), v2 = ntload(vp + 64 * (k + 2)),
+              v3 = ntload(vp + 64 * (k + 3));
       acc += v0 * x[sell_col(c0, W)];
       acc += v1 * x[sell_col(c1, W)];
       acc += v2 * x[sell_col(c2, W)];
       acc += v3 * x[sell_col(c3, W)];
     }
-    for (; k < w; ++k) acc += vp[64 * k] * x[sell_col(cp[64 * k], W)];
+    for (; k < w; ++k) acc += ntload(vp + 64 * k) * x[sell_col(ntload(cp + 64 * k), W)];
   } else {
     const int32_t* cp = S.sc + off + lane;
     int k = 0;
     for (; k + 4 <= w; k += 4) {
-      const int c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)], c3 = cp[64 * (k + 3)];
-      const T v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)], v3 = vp[64 * (k + 3)];
+      const int c0 = ntload(cp + 64 * k), c1 = ntload(cp + 64 * (k + 1)), c2 = ntload(cp + 64 * (k + 2)),
+                c3 = ntload(cp + 64 * (k + 3));
+      const T v0 = ntload(vp + 64 * k), v1 = ntload(vp + 64 * (k + 1)), v2 = ntload(vp + 64 * (k + 2)),
+              v3 = ntload(vp + 64 * (k + 3));
       acc += v0 * x[c0];
       acc += v1 * x[c1];
       acc += v2 * x[c2];
       acc += v3 * x[c3];
     }
-    for (; k < w; ++k) acc += vp[64 * k] * x[cp[64 * k]];
+    for (; k < w; ++k) acc += ntload(vp + 64 * k) * x[ntload(cp + 64 * k)];
   }
   return acc;
 }
@@ -164,7 +188,8 @@ __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, S
                                                  const T* __restrict__ b, const T* __restrict__ dinv,
                                                  T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
   // the slice index is the same on every lane of the wavefront
-  const int slice = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kNT / 64) + (threadIdx.x >> 6)));
+  const int slice =
+      __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
   const T acc = sell_row_sum<T>(S, slice, lane, sv, x);

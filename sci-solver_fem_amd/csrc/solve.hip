@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "fem_internal.hpp"
 #include "kernels.hpp"
@@ -146,6 +147,55 @@ __global__ void scale_dinv_kernel(int64_t n, int k, const V* __restrict__ dinv, 
   if (d) d[t] = v;
 }
 
+// First level-0 smoothing step of a mixed-precision V-cycle, from the FP64
+// Krylov residual: b = fl32(r); x = c1 * dinv * b; d = x if d != null.
+__global__ void scale_in_kernel(int64_t n, const float* __restrict__ dinv, const double* __restrict__ r,
+                                float* __restrict__ b, float* __restrict__ x, float* __restrict__ d, float c1) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float bv = (float)r[i];
+  b[i] = bv;
+  const float v = c1 * dinv[i] * bv;
+  x[i] = v;
+  if (d) d[i] = v;
+}
+
+// Last level-0 smoothing step of a mixed-precision V-cycle (one column,
+// sliced ELL): the smoothed value goes straight to the FP64 output z, with
+// per-block partials of r.z for the PCG beta (d is not needed afterwards).
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_smooth_out_kernel(int64_t n, int64_t nslices, SellView S,
+                                                            const float* __restrict__ sv, const float* __restrict__ x,
+                                                            const float* __restrict__ b, const float* __restrict__ dinv,
+                                                            const float* __restrict__ d, float c1, float c2,
+                                                            const double* __restrict__ r, double* __restrict__ z,
+                                                            double* __restrict__ partials) {
+  __shared__ double s_buf[kNT];
+  const int slice =
+      __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  if (slice < nslices) {
+    const float acc = sell_row_sum<float>(S, slice, lane, sv, x);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row < n) {
+      float xn;
+      if (MODE == kJacobi) {
+        xn = x[row] + c1 * dinv[row] * (b[row] - acc);
+      } else {
+        const float rr = ntload(dinv + row) * (ntload(b + row) - acc);
+        const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * rr;
+        xn = x[row] + dn;
+      }
+      const double zv = (double)xn;
+      z[row] = zv;
+      part = r[row] * zv;
+    }
+  }
+  const double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
 template <typename Tin, typename Tout>
 __global__ void convert_kernel(int64_t m, const Tin* __restrict__ a, Tout* __restrict__ b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,11 +231,14 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int lane = threadIdx.x & (G - 1);
-  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / G);
-  for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / G; row < n; row += rows_per_grid) {
+  // block (XCD-remapped) owns a contiguous range of rows, kNT / G per pass
+  const int64_t per = kNT / G;
+  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
+  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
+  for (int64_t row = r0 + threadIdx.x / G; row < r1; row += per) {
     const int s = rp[row], e = rp[row + 1];
     double acc = 0.0;
-    for (int j = s + lane; j < e; j += G) acc += val[j] * p[ci[j]];
+    for (int j = s + lane; j < e; j += G) acc += ntload(val + j) * p[ntload(ci + j)];
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
     if (lane == 0) {
@@ -198,20 +251,21 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
 }
 
 // q = A p on the sliced-ELL copy and per-block partials of p.q (one column);
-// blocks stride over groups of 4 slices so the partial count stays <= grid.
+// a wavefront per slice as in the smoother kernels (no grid-stride loop).
 __global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t nslices, SellView S,
                                                          const double* __restrict__ sv, const double* __restrict__ p,
                                                          double* __restrict__ q, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
   const int lane = threadIdx.x & 63;
+  const int slice =
+      __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
   double part = 0.0;
-  for (int slice = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kNT / 64) + (threadIdx.x >> 6)));
-       slice < nslices; slice += gridDim.x * (kNT / 64)) {
+  if (slice < nslices) {
     const double acc = sell_row_sum<double>(S, slice, lane, sv, p);
     const int64_t row = (int64_t)slice * 64 + lane;
     if (row < n) {
       q[row] = acc;
-      part += acc * p[row];
+      part = acc * p[row];
     }
   }
   double tot = block_sum(part, s_buf);
@@ -227,12 +281,15 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
-  const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
+  // block (XCD-remapped) owns a contiguous range of rows, kNT / KP per pass
+  const int64_t per = kNT / KP;
+  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
+  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
   if (c < k) {
-    for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid) {
+    for (int64_t row = r0 + threadIdx.x / KP; row < r1; row += per) {
       const int s = rp[row], e = rp[row + 1];
       double acc = 0.0;
-      for (int j = s; j < e; ++j) acc += val[j] * p[(int64_t)ci[j] * k + c];
+      for (int j = s; j < e; ++j) acc += ntload(val + j) * p[(int64_t)ntload(ci + j) * k + c];
       q[row * k + c] = acc;
       part += acc * p[row * k + c];
     }
@@ -280,14 +337,22 @@ __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const do
 enum FinishMode { F_STORE = 0, F_ALPHA = 1, F_RR = 2, F_BETA = 3 };
 
 // one block: column totals of nb partials, then the scalar update
-__global__ void __launch_bounds__(kNT) pcg_finish_kernel(int nb, int k, const double* __restrict__ partials,
-                                                       double* __restrict__ scal, int mode, int slot, double tol2) {
-  __shared__ double s_buf[kNT];
+// 1024 threads so that ~16k partials take a few rounds of independent loads.
+constexpr int kFinNT = 1024;
+
+__global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const double* __restrict__ partials,
+                                                          double* __restrict__ scal, int mode, int slot, double tol2) {
+  __shared__ double s_buf[kFinNT / 64];
   for (int c = 0; c < k; ++c) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kNT) v += partials[(int64_t)b * k + c];
-    v = block_sum(v, s_buf);
+#pragma unroll 8
+    for (int b = threadIdx.x; b < nb; b += kFinNT) v += partials[(int64_t)b * k + c];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s_buf[threadIdx.x >> 6] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+      v = 0.0;
+      for (int w = 0; w < kFinNT / 64; ++w) v += s_buf[w];
       if (mode == F_STORE) {
         scal[slot * k + c] = v;
       } else if (mode == F_ALPHA) {
@@ -303,6 +368,7 @@ __global__ void __launch_bounds__(kNT) pcg_finish_kernel(int nb, int k, const do
         scal[S_RZ * k + c] = v;
       }
     }
+    __syncthreads();
   }
 }
 
@@ -354,10 +420,37 @@ double matrix_bytes(const Csr& A, int k) {
   return (double)A.nnz * sizeof(T) + 2.0 * nc + 4.0 * ((double)A.nnz - nc);
 }
 
+template <typename T>
+void fill_zero(T* p, int64_t n, hipStream_t s);
+template <>
+void fill_zero<double>(double* p, int64_t n, hipStream_t s) { dev_fill_f64(p, 0.0, (size_t)n, s); }
+template <>
+void fill_zero<float>(float* p, int64_t n, hipStream_t s) { dev_fill_f32(p, 0.0f, (size_t)n, s); }
+
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
 // Buffers x and t ping-pong; returns the buffer that holds the result.
+// Fused FP64 ends of a mixed-precision level-0 smoother (one column, sliced
+// ELL): the first step reads the Krylov residual r64 (and stores fl32(r64) in
+// b32 for the later steps), the last step writes z64 and r64.z64 partials.
+struct Level0Io {
+  const double* r64 = nullptr;
+  float* b32 = nullptr;
+  double* z64 = nullptr;
+  double* partials = nullptr;
+  unsigned nb = 0;  // partials written by the last step
+};
+
 template <typename T>
-T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, bool zero, int k) {
+bool fused_out(const Level0Io* io, const Csr& A, int k) {
+  return std::is_same<T, float>::value && io && io->z64 && k == 1 && A.nslices > 0 && A.s32.size() > 0;
+}
+
+// nu steps of the smoother from x = 0 (zero == true) or from the given x.
+// Buffers x and t ping-pong; returns the buffer that holds the result (with
+// io->z64 set, the last step writes there instead and nullptr is returned).
+template <typename T>
+T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, bool zero, int k,
+          Level0Io* io = nullptr) {
   Level& L = *h.H.levels[l];
   hipStream_t s = h.stream;
   const T* dinv = dinv_of<T>(L);
@@ -367,21 +460,49 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   // (Jacobi: x, b, dinv, x_new; Chebyshev: + d read and write)
   const double sweep_bytes = matrix_bytes<T>(A, k) + (double)A.n * sizeof(T) * (S.type ? 6.0 : 4.0) * k;
   if (nu <= 0) {
-    if (zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(T) * A.n * k, s));
+    // a fill kernel, not a memset: this may run inside a captured graph
+    if (zero) fill_zero<T>(x, A.n * k, s);
     return x;
   }
+  const bool in64 = zero && io && io->r64 && std::is_same<T, float>::value && k == 1;
+  const bool out64 = !zero && fused_out<T>(io, A, k);
+  auto first_step = [&](T* xo, T* dout, double c) {
+    if (in64) {
+      scale_in_kernel<<<grid_for(A.n, 256), 256, 0, s>>>(A.n, (const float*)dinv, io->r64, io->b32, (float*)xo,
+                                                          (float*)dout, (float)c);
+    } else {
+      scale_dinv_kernel<T, T, T><<<grid_for(A.n * k, 256), 256, 0, s>>>(A.n, k, dinv, b, xo, dout, (T)c);
+    }
+    HIP_LAUNCH_CHECK();
+  };
+  auto last_step = [&](int mode, T* cur, double c1, double c2) {
+    const unsigned g = grid_for(A.nslices, kNT / 64);
+    if (mode == kJacobi)
+      sell_smooth_out_kernel<kJacobi><<<g, kNT, 0, s>>>(A.n, A.nslices, sell_view(A), A.s32.get(), (const float*)cur,
+                                                         (const float*)b, (const float*)dinv, (const float*)d,
+                                                         (float)c1, (float)c2, io->r64, io->z64, io->partials);
+    else
+      sell_smooth_out_kernel<kCheb><<<g, kNT, 0, s>>>(A.n, A.nslices, sell_view(A), A.s32.get(), (const float*)cur,
+                                                       (const float*)b, (const float*)dinv, (const float*)d, (float)c1,
+                                                       (float)c2, io->r64, io->z64, io->partials);
+    HIP_LAUNCH_CHECK();
+    io->nb = g;
+  };
   T* cur = x;
   T* nxt = t;
   int step = 0;
   if (S.type == 0) {
     const T w = (T)S.omega;
     if (zero) {
-      scale_dinv_kernel<T, T, T><<<grid_for(A.n * k, 256), 256, 0, s>>>(A.n, k, dinv, b, cur, (T*)nullptr, w);
-      HIP_LAUNCH_CHECK();
+      first_step(cur, nullptr, S.omega);
       step = 1;
     }
     for (; step < nu; ++step) {
       ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
+      if (out64 && step == nu - 1) {
+        last_step(kJacobi, cur, S.omega, 0.0);
+        return nullptr;
+      }
       spmv_op<T, kJacobi>(A, k, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
       T* tmp = cur; cur = nxt; nxt = tmp;
     }
@@ -392,8 +513,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   if (zero) {
-    scale_dinv_kernel<T, T, T><<<grid_for(A.n * k, 256), 256, 0, s>>>(A.n, k, dinv, b, cur, d, (T)(1.0 / theta));
-    HIP_LAUNCH_CHECK();
+    first_step(cur, d, 1.0 / theta);
     step = 1;
   }
   for (; step < nu; ++step) {
@@ -408,6 +528,10 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
       rho_old = rho_new;
     }
     ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
+    if (out64 && step == nu - 1) {
+      last_step(kCheb, cur, c1, c2);
+      return nullptr;
+    }
     spmv_op<T, kCheb>(A, k, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
     T* tmp = cur; cur = nxt; nxt = tmp;
   }
@@ -415,9 +539,9 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
 }
 
 // V-cycle below level l for the right-hand side b (level l vectors, k columns);
-// returns the buffer holding the correction.
+// returns the buffer holding the correction (nullptr when io->z64 received it).
 template <typename T>
-T* vcycle(Handle& h, int l, const T* b, int k) {
+T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr) {
   Hierarchy& H = h.H;
   Level& L = *H.levels[l];
   hipStream_t s = h.stream;
@@ -433,7 +557,7 @@ T* vcycle(Handle& h, int l, const T* b, int k) {
   T* d = LV<T>::d(L);
   const int nu_pre = h.prm.preInnerIters;
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
-  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k);
+  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k, io);
   T* oth = (cur == x) ? t : x;
   {
     ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", matrix_bytes<T>(A, k) + 3.0 * A.n * k * sizeof(T));
@@ -444,12 +568,16 @@ T* vcycle(Handle& h, int l, const T* b, int k) {
   spmv_op<T, kAssign>(L.R, k, r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   T* xc = vcycle<T>(h, l + 1, bc, k);
   spmv_op<T, kAdd>(L.P, k, xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
-  cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k);
+  cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k, io);
   return cur;
 }
 
-// z = M r at level 0 (FP64 in and out); k columns
-void precondition(Handle& h, const double* r, double* z, int k) {
+// z = M r at level 0 (FP64 in and out); k columns.  With `partials` (one
+// column, mixed precision, level 0 smoothed before and after the coarse
+// correction, sliced ELL) the conversions are fused into the first and last
+// level-0 smoothing steps and the last one also writes per-block partials of
+// r.z there: the return value is their count (0 when not fused).
+unsigned precondition(Handle& h, const double* r, double* z, int k, double* partials = nullptr) {
   hipStream_t s = h.stream;
   Level& L0 = *h.H.levels[0];
   const int64_t m = h.A.n * k;
@@ -457,13 +585,27 @@ void precondition(Handle& h, const double* r, double* z, int k) {
     double* res = vcycle<double>(h, 0, r, k);
     copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
     HIP_LAUNCH_CHECK();
-  } else {
-    convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
-    HIP_LAUNCH_CHECK();
-    float* res = vcycle<float>(h, 0, L0.b32.get(), k);
-    convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
-    HIP_LAUNCH_CHECK();
+    return 0;
   }
+  const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
+  const bool fuse = partials && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
+                    h.A.nslices > 0 && h.A.s32.size() > 0;
+  if (fuse) {
+    Level0Io io;
+    io.r64 = r;
+    io.b32 = L0.b32.get();
+    io.z64 = z;
+    io.partials = partials;
+    float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io);
+    if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
+    return io.nb;
+  }
+  convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
+  HIP_LAUNCH_CHECK();
+  float* res = vcycle<float>(h, 0, L0.b32.get(), k);
+  convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
+  HIP_LAUNCH_CHECK();
+  return 0;
 }
 
 }  // namespace
@@ -533,6 +675,7 @@ void apply_operator_int(Handle& h, const double* x, double* y, int k) {
 void ensure_krylov(Handle& h, int k) {
   const int64_t n = h.A.n;
   if (h.kcap < k) {
+    ++h.gen;
     h.kx.resize(n * k); h.kr.resize(n * k); h.kp.resize(n * k); h.kq.resize(n * k);
     h.kz.resize(n * k); h.kb.resize(n * k);
     h.ubuf.resize(n * k); h.ubuf2.resize(n * k);
@@ -544,6 +687,7 @@ void ensure_krylov(Handle& h, int k) {
   for (auto& Lp : h.H.levels) {
     Level& L = *Lp;
     if (L.wk >= k) continue;
+    ++h.gen;
     const int64_t nl = (&L == h.H.levels[0].get()) ? h.A.n : L.A.n;
     if (h.H.precision == 0) {
       L.x64.resize(nl * k); L.t64.resize(nl * k); L.b64.resize(nl * k); L.r64.resize(nl * k); L.d64.resize(nl * k);
@@ -558,8 +702,45 @@ void apply_vcycle_fp64(Handle& h, const double* r, double* z) { precondition(h, 
 
 namespace {
 
+// Runs `body` (a fixed sequence of launches on h.stream that reads and writes
+// device memory only) either directly or as a HIP graph captured at its first
+// use and replayed while the buffers, hierarchy, k and tolerance it was
+// captured with are unchanged.  Replays launch the same kernels with the same
+// arguments, so results are bitwise those of the direct launches.
+template <typename Fn>
+void run_iteration(Handle& h, int k, double tol2, Fn&& body) {
+  if (h.prm.launchMode == 0 || h.profiling || h.stream == nullptr) {
+    body();
+    return;
+  }
+  if (h.iter_graph && (h.graph_gen != h.gen || h.graph_k != k || h.graph_tol2 != tol2)) {
+    HIP_CHECK(hipGraphExecDestroy(h.iter_graph));
+    h.iter_graph = nullptr;
+  }
+  if (!h.iter_graph) {
+    HIP_CHECK(hipStreamBeginCapture(h.stream, hipStreamCaptureModeRelaxed));
+    try {
+      body();
+    } catch (...) {
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(h.stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamEndCapture(h.stream, &g));
+    hipError_t e = hipGraphInstantiate(&h.iter_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_CHECK(e);
+    h.graph_gen = h.gen;
+    h.graph_k = k;
+    h.graph_tol2 = tol2;
+  }
+  HIP_CHECK(hipGraphLaunch(h.iter_graph, h.stream));
+}
+
 void finish(Handle& h, int nb, int k, int mode, int slot, double tol2 = 0.0) {
-  pcg_finish_kernel<<<1, kNT, 0, h.stream>>>(nb, k, h.partials.get(), h.scal.get(), mode, slot, tol2);
+  pcg_finish_kernel<<<1, kFinNT, 0, h.stream>>>(nb, k, h.partials.get(), h.scal.get(), mode, slot, tol2);
   HIP_LAUNCH_CHECK();
 }
 
@@ -609,7 +790,7 @@ void update(Handle& h, int k, double tol2) {
 
 void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
-  unsigned g = std::min<unsigned>(grid_for(A.nslices, kNT / 64), 2048u);
+  unsigned g = grid_for(A.nslices, kNT / 64);
   pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(),
                                                h.partials.get());
   HIP_LAUNCH_CHECK();
@@ -727,22 +908,38 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     set_scal(h, S_ACT, k, act.data());
     int restarts = 0;
     while (any) {
-      precondition(h, h.kr.get(), h.kz.get(), k);
+      // z = M r and rz = r.z (fused into the last level-0 smoothing step when possible)
+      auto precondition_dot = [&](int mode, int slot) {
+        const unsigned nb = precondition(h, h.kr.get(), h.kz.get(), k, h.partials.get());
+        if (nb) finish(h, (int)nb, k, mode, slot);
+        else dot(h, k, h.kr.get(), h.kz.get(), mode, slot);
+      };
+      precondition_dot(F_STORE, S_RZ);
       copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, h.kz.get(), h.kp.get());
       HIP_LAUNCH_CHECK();
-      dot(h, k, h.kr.get(), h.kz.get(), F_STORE, S_RZ);
+      // the first iteration starts from p = z; every later one is the fixed
+      // sequence below (captured once, then replayed)
+      auto iteration = [&]() {
+        precondition_dot(F_BETA, 0);  // z, beta, rz
+        pcg_p_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, h.kz.get(), h.kp.get(), h.scal.get());
+        HIP_LAUNCH_CHECK();
+        spmv_dot(h, k);      // q = A p, alpha
+        update(h, k, tol2);  // x, r, rr, act
+      };
+      bool first = true;
       while (iters < h.prm.maxIters) {
-        spmv_dot(h, k);     // q = A p, alpha
-        update(h, k, tol2); // x, r, rr, act
+        if (first) {
+          spmv_dot(h, k);
+          update(h, k, tol2);
+          first = false;
+        } else {
+          run_iteration(h, k, tol2, iteration);
+        }
         ++iters;
         get_scal(h, S_ACT, k, act.data());
         bool still = false;
         for (int c = 0; c < k; ++c) still = still || act[c] != 0.0;
         if (!still) break;
-        precondition(h, h.kr.get(), h.kz.get(), k);
-        dot(h, k, h.kr.get(), h.kz.get(), F_BETA, 0);  // beta, rz
-        pcg_p_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, h.kz.get(), h.kp.get(), h.scal.get());
-        HIP_LAUNCH_CHECK();
       }
       // verify on the true residual; restart columns that drifted
       true_residual(rel);

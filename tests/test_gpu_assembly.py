@@ -100,3 +100,31 @@ def test_element_grad_matches_oracle(fem, gpu, oracle, meshgen):
         og = oracle.element_grad(xyz, conn, a, u)
         assert np.abs(g - og).max() <= 1e-12 * np.abs(og).max()
         s.close()
+
+
+def test_invalid_mesh_inputs_rejected(fem, gpu, meshgen):
+    """Out-of-range vertex indices and NaN inputs are rejected with a status
+    (checked on the device before anything is built), never a crash."""
+    report("[gpu] invalid mesh inputs")
+    xyz, tets = meshgen.cube_mesh(5, seed=1)
+    kappa = meshgen.lognormal_kappa(1, tets.shape[0], 1.0)
+    bad_conn = tets.copy()
+    bad_conn[7, 2] = xyz.shape[0]
+    with pytest.raises(fem.FemError) as e:
+        fem.FEMSolver.from_arrays(3, xyz, bad_conn, kappa=kappa, mass_coeff=1e-2)
+    assert e.value.status == 1 and "out of range" in str(e.value)
+    bad_k = kappa.copy()
+    bad_k[3] = np.nan
+    with pytest.raises(fem.FemError) as e:
+        fem.FEMSolver.from_arrays(3, xyz, tets, kappa=bad_k, mass_coeff=1e-2)
+    assert e.value.status == 6
+    bad_x = xyz.copy()
+    bad_x[11, 1] = np.nan
+    with pytest.raises(fem.FemError) as e:
+        fem.FEMSolver.from_arrays(3, bad_x, tets, kappa=kappa, mass_coeff=1e-2)
+    assert e.value.status == 6
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    with pytest.raises(fem.FemError) as e:
+        s.update_kappa(bad_k)
+    assert e.value.status == 6
+    s.close()

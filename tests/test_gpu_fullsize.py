@@ -1,0 +1,117 @@
+"""GPU runs at the benchmark configurations' full sizes, checked through
+properties that do not need a direct solve: the true FP64 residual evaluated
+on the host from the exported matrix, agreement of the mixed and FP64 paths,
+linearity, the closed form 1^T K 1 = mass_coeff * vol(domain), symmetry, and
+the sliced-ELL operator (both its 16-bit-column and int32-column slices)
+against a host CSR product."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import report
+
+pytestmark = pytest.mark.gpu
+
+# bench.py's V-cycle settings
+PARAMS = dict(solverType=1, tolerance=1e-8, maxIters=500, smootherType=1, preInnerIters=2, postInnerIters=2,
+              postRelaxes=1, topSize=256, aggregationRule=1)
+
+
+def _config(meshgen, name):
+    c = meshgen.CONFIGS[name]
+    xyz, tets = meshgen.cube_mesh(c["N"], seed=0)
+    kappa = meshgen.lognormal_kappa(0, tets.shape[0], c["sigma"])
+    return xyz, tets, kappa, c["mass"]
+
+
+def _true_residual(A, x, b):
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+def _check_mass_closed_form(A, mass):
+    """Stiffness rows sum to zero and the P1 mass matrix sums to vol = 1, so
+    1^T K 1 = mass_coeff up to the rounding of the assembled entries."""
+    one = np.ones(A.shape[0])
+    scale = np.abs(A.data).sum()
+    assert abs(one @ (A @ one) - mass) <= 1e-13 * scale + 1e-12 * mass
+
+
+def _wide_slices(rp, ci, bits=13, windows=8):
+    """Number of 64-row slices whose columns touch more than `windows`
+    aligned blocks of 2^bits columns (stored with int32 columns)."""
+    n = rp.size - 1
+    wide = 0
+    for s0 in range(0, n, 64):
+        blk = np.unique(ci[rp[s0]:rp[min(n, s0 + 64)]] >> bits)
+        wide += blk.size > windows
+    return wide, (n + 63) // 64
+
+
+@pytest.mark.parametrize("name", ["c2_cube49_fp64", "c3_cube97_mixed"])
+def test_config_solves(fem, gpu, meshgen, name):
+    report(f"[gpu] full-size {name}")
+    xyz, tets, kappa, mass = _config(meshgen, name)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    A = s.get_matrix_scipy()
+    assert (A != A.T).nnz == 0
+    _check_mass_closed_form(A, mass)
+    b = meshgen.gaussian_rhs(0, xyz.shape[0])
+    xs = {}
+    for prec in (0, 1):
+        s.set_params(precision=prec, **PARAMS)
+        x, info = s.solve(b)
+        res = _true_residual(A, x, b)
+        report(f"      precision={prec}: {info['iterations']} iterations, host residual {res:.2e}, "
+               f"{info['num_levels']} levels")
+        assert info["status"] == 0 and info["converged"] == 1
+        assert res <= 1e-8 * (1 + 1e-6)
+        xs[prec] = x
+    # both paths stop at 1e-8 on the same operator: the error bound is cond(A)
+    # times the residual; the observed difference is far below that
+    diff = np.linalg.norm(xs[1] - xs[0]) / np.linalg.norm(xs[0])
+    report(f"      |x_mixed - x_fp64| / |x_fp64| = {diff:.2e}")
+    assert diff <= 1e-6
+    s.close()
+
+
+def test_c3_operator_both_slice_kinds(fem, gpu, meshgen):
+    report("[gpu] 97^3 sliced-ELL operator vs host CSR")
+    xyz, tets, kappa, mass = _config(meshgen, "c3_cube97_mixed")
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    s.set_params(precision=1, **PARAMS)
+    s.setup()
+    rp, ci, _, _ = s.level_matrix(0)
+    wide, ns = _wide_slices(rp, ci)
+    report(f"      {ns - wide} compact / {wide} int32-column slices")
+    assert 0 < wide < ns
+    A = s.get_matrix_scipy()
+    rng = np.random.default_rng(97)
+    X = rng.normal(size=(A.shape[0], 2))
+    for c in range(2):
+        y = s.apply_operator(np.ascontiguousarray(X[:, c]))
+        ref = A @ X[:, c]
+        err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+        assert err <= 1e-13, err
+    s.close()
+
+
+def test_c5_full_size_properties(fem, gpu, meshgen):
+    report("[gpu] 161^3 (headline) mixed solve, linearity")
+    xyz, tets, kappa, mass = _config(meshgen, "c5_cube161_mixed16")
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    s.set_params(precision=1, **PARAMS)
+    A = s.get_matrix_scipy()
+    _check_mass_closed_form(A, mass)
+    B = meshgen.gaussian_rhs(0, xyz.shape[0], 2)
+    b1, b2 = np.ascontiguousarray(B[:, 0]), np.ascontiguousarray(B[:, 1])
+    x1, i1 = s.solve(b1)
+    x2, _ = s.solve(b2)
+    x3, _ = s.solve(b1 + 2.0 * b2)
+    for x, b in ((x1, b1), (x2, b2), (x3, b1 + 2.0 * b2)):
+        res = _true_residual(A, x, b)
+        assert res <= 1e-8 * (1 + 1e-6), res
+    lin = np.linalg.norm(x3 - (x1 + 2.0 * x2)) / np.linalg.norm(x3)
+    report(f"      {i1['iterations']} iterations, {i1['num_levels']} levels, linearity error {lin:.2e}")
+    assert lin <= 1e-6
+    s.close()

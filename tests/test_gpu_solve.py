@@ -171,3 +171,37 @@ def test_reference_sanity2d(fem, gpu):
     ans = fem.read_matlab_array(str(REFDATA / "simpleTriAns.mat"))
     assert np.linalg.norm(x - ans) < 100.0
     cfg.close()
+
+
+def test_graph_replay_matches_eager(fem, gpu, oracle, meshgen):
+    """launchMode 1 replays one captured PCG iteration; it must give bitwise the
+    results of launching every kernel (launchMode 0): across repeated solves,
+    a tolerance change (re-capture), k = 2 columns and a re-setup."""
+    report("[gpu] captured PCG iteration vs eager launches")
+    xyz, conn, kappa, A, b = _problem(meshgen, oracle, 17, 1.5, 1e-2, seed=4)
+    B = meshgen.gaussian_rhs(9, xyz.shape[0], 2)
+    runs = {}
+    for mode in (0, 1):
+        s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+        s.set_params(solverType=1, tolerance=1e-9, maxIters=300, precision=1, smootherType=1, preInnerIters=2,
+                     postInnerIters=2, topSize=64, launchMode=mode)
+        out = []
+        for _ in range(3):
+            x, info = s.solve(b)
+            out.append((x, info["iterations"]))
+        s.set_params(tolerance=1e-11)
+        x, info = s.solve(b)
+        out.append((x, info["iterations"]))
+        X, info = s.solve(B)
+        out.append((X, info["iterations"]))
+        s.set_params(precision=0)  # new hierarchy
+        x, info = s.solve(b)
+        out.append((x, info["iterations"]))
+        runs[mode] = out
+        s.close()
+    for (x0, i0), (x1, i1) in zip(runs[0], runs[1]):
+        assert i0 == i1
+        assert np.array_equal(x0, x1)
+    # tolerance 1e-11 on the device residual; the host re-evaluation of b - A x
+    # carries ~|A||x| eps of rounding of its own (x ~ 1e4 here)
+    assert np.linalg.norm(b - A @ runs[1][3][0]) / np.linalg.norm(b) <= 1e-10
