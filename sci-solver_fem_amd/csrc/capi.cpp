@@ -266,6 +266,7 @@ void fem_destroy(fem_handle* hh) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->h_scal) (void)hipHostFree(h->h_scal);
+  if (h->h_act) (void)hipHostFree(h->h_act);
   if (h->iter_graph) (void)hipGraphExecDestroy(h->iter_graph);
   release_profile_events(*h);
   hipStream_t s = h->own_stream ? h->stream : nullptr;

@@ -129,6 +129,8 @@ struct Handle {
   DevBuf<double> partials;        // reduction partials
   DevBuf<double> scal;            // device scalars
   double* h_scal = nullptr;       // pinned mirror of scal
+  double* h_act = nullptr;        // per-column convergence flags, host memory mapped
+  double* d_act = nullptr;        //   for the device (written by the |r|^2 finish)
   DevBuf<double> ubuf, ubuf2;     // user-order staging (nv * k)
 
   // captured PCG iteration (launchMode 1), valid while gen == graph_gen

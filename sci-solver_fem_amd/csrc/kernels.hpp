@@ -103,7 +103,10 @@ __global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const 
   if (row >= n || c >= k) return;
   const int s = rp[row], e = rp[row + 1];
   T acc = T(0);
-  for (int q = s; q < e; ++q) acc += (T)ntload(val + q) * x[(int64_t)ntload(ci + q) * k + c];
+  // plain loads: the KP lanes of a row walk its entries one by one, so each
+  // cache line of val / ci is used over several iterations (a nontemporal
+  // hint would evict it after the first)
+  for (int q = s; q < e; ++q) acc += (T)val[q] * x[(int64_t)ci[q] * k + c];
   spmv_epilogue<MODE, T, V>(row * k + c, row, acc, y, x, b, dinv, d, xn, c1, c2);
 }
 

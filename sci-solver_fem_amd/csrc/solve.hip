@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
     for (int64_t row = r0 + threadIdx.x / KP; row < r1; row += per) {
       const int s = rp[row], e = rp[row + 1];
       double acc = 0.0;
-      for (int j = s; j < e; ++j) acc += ntload(val + j) * p[(int64_t)ntload(ci + j) * k + c];
+      for (int j = s; j < e; ++j) acc += val[j] * p[(int64_t)ci[j] * k + c];
       q[row * k + c] = acc;
       part += acc * p[row * k + c];
     }
@@ -341,7 +341,8 @@ enum FinishMode { F_STORE = 0, F_ALPHA = 1, F_RR = 2, F_BETA = 3 };
 constexpr int kFinNT = 1024;
 
 __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const double* __restrict__ partials,
-                                                          double* __restrict__ scal, int mode, int slot, double tol2) {
+                                                          double* __restrict__ scal, int mode, int slot, double tol2,
+                                                          double* __restrict__ host_act) {
   __shared__ double s_buf[kFinNT / 64];
   for (int c = 0; c < k; ++c) {
     double v = 0.0;
@@ -362,6 +363,7 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
         scal[S_RR * k + c] = v;
         bool act = scal[S_ACT * k + c] != 0.0 && v > tol2 * scal[S_BB * k + c];
         scal[S_ACT * k + c] = act ? 1.0 : 0.0;
+        if (host_act) host_act[c] = act ? 1.0 : 0.0;  // read by the host once the stream drains
       } else {
         double rz = scal[S_RZ * k + c];
         scal[S_BETA * k + c] = (scal[S_ACT * k + c] != 0.0 && rz != 0.0) ? v / rz : 0.0;
@@ -459,6 +461,8 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   // algorithmic bytes of one sweep: the matrix and the per-row vectors
   // (Jacobi: x, b, dinv, x_new; Chebyshev: + d read and write)
   const double sweep_bytes = matrix_bytes<T>(A, k) + (double)A.n * sizeof(T) * (S.type ? 6.0 : 4.0) * k;
+  // fused last step: x, b, dinv (+ d) in FP32, r read and z written in FP64
+  const double out_bytes = matrix_bytes<T>(A, k) + (double)A.n * (4.0 * (S.type ? 4.0 : 3.0) + 16.0);
   if (nu <= 0) {
     // a fill kernel, not a memset: this may run inside a captured graph
     if (zero) fill_zero<T>(x, A.n * k, s);
@@ -498,11 +502,12 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
       step = 1;
     }
     for (; step < nu; ++step) {
-      ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
       if (out64 && step == nu - 1) {
+        ProfScope ps(h, prof ? "smooth_l0_out" : "", out_bytes);
         last_step(kJacobi, cur, S.omega, 0.0);
         return nullptr;
       }
+      ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
       spmv_op<T, kJacobi>(A, k, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
       T* tmp = cur; cur = nxt; nxt = tmp;
     }
@@ -527,11 +532,12 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
       c2 = 2.0 * rho_new / delta;
       rho_old = rho_new;
     }
-    ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
     if (out64 && step == nu - 1) {
+      ProfScope ps(h, prof ? "smooth_l0_out" : "", out_bytes);
       last_step(kCheb, cur, c1, c2);
       return nullptr;
     }
+    ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
     spmv_op<T, kCheb>(A, k, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
     T* tmp = cur; cur = nxt; nxt = tmp;
   }
@@ -682,6 +688,10 @@ void ensure_krylov(Handle& h, int k) {
     h.partials.resize((size_t)std::max<int64_t>(4096, n / 256 + 2) * 64);
     h.scal.resize((size_t)S_COUNT * 64);
     if (!h.h_scal) HIP_CHECK(hipHostMalloc((void**)&h.h_scal, sizeof(double) * S_COUNT * 64, hipHostMallocDefault));
+    if (!h.h_act) {
+      HIP_CHECK(hipHostMalloc((void**)&h.h_act, sizeof(double) * 64, hipHostMallocMapped | hipHostMallocCoherent));
+      HIP_CHECK(hipHostGetDevicePointer((void**)&h.d_act, h.h_act, 0));
+    }
     h.kcap = k;
   }
   for (auto& Lp : h.H.levels) {
@@ -740,7 +750,8 @@ void run_iteration(Handle& h, int k, double tol2, Fn&& body) {
 }
 
 void finish(Handle& h, int nb, int k, int mode, int slot, double tol2 = 0.0) {
-  pcg_finish_kernel<<<1, kFinNT, 0, h.stream>>>(nb, k, h.partials.get(), h.scal.get(), mode, slot, tol2);
+  pcg_finish_kernel<<<1, kFinNT, 0, h.stream>>>(nb, k, h.partials.get(), h.scal.get(), mode, slot, tol2,
+                                                 mode == F_RR ? h.d_act : nullptr);
   HIP_LAUNCH_CHECK();
 }
 
@@ -848,6 +859,19 @@ void set_scal(Handle& h, int slot, int k, const double* v) {
   HIP_CHECK(hipMemcpyAsync(h.scal.get() + slot * k, v, sizeof(double) * k, hipMemcpyHostToDevice, h.stream));
 }
 
+// Wait for the stream by polling (no sleep / wake-up between PCG iterations)
+// and read the convergence flags the last |r|^2 finish wrote to mapped host
+// memory: no copy is queued.
+void wait_act(Handle& h, int k, double* v) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(h.stream);
+    if (e == hipSuccess) break;
+    if (e != hipErrorNotReady) HIP_CHECK(e);
+  }
+  const volatile double* a = h.h_act;
+  for (int c = 0; c < k; ++c) v[c] = a[c];
+}
+
 void get_scal(Handle& h, int slot, int k, double* v) {
   HIP_CHECK(hipMemcpyAsync(h.h_scal, h.scal.get() + slot * k, sizeof(double) * k, hipMemcpyDeviceToHost, h.stream));
   HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -936,7 +960,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
           run_iteration(h, k, tol2, iteration);
         }
         ++iters;
-        get_scal(h, S_ACT, k, act.data());
+        wait_act(h, k, act.data());
         bool still = false;
         for (int c = 0; c < k; ++c) still = still || act[c] != 0.0;
         if (!still) break;
