@@ -747,11 +747,12 @@ void build_hierarchy(Handle& h) {
   // Sliced-ELL copies for the solve phase, for operators with short rows (the
   // fine level, prolongators).  Coarse operators have long rows (~50-300) for
   // which a lane per row serialises; they keep the multi-lane CSR kernels.
-  auto sell_or_csr = [&](Csr& M) {
+  auto sell_or_csr = [&](Csr& M, int64_t sort_window) {
     if (M.n > 0 && (double)M.nnz / (double)M.n <= 24.0) {
-      build_sell(M, H.precision == 1, s);
+      build_sell(M, H.precision == 1, s, sort_window);
     } else {
       M.nslices = 0;
+      M.srow.release();
       M.sp.release();
       M.sc.release();
       M.s64.release();
@@ -760,8 +761,10 @@ void build_hierarchy(Handle& h) {
   };
   for (size_t l = 0; l < H.levels.size(); ++l) {
     Level& L = *H.levels[l];
-    sell_or_csr((l == 0) ? h.A : L.A);
-    if ((int)l != H.coarse) sell_or_csr(L.P);
+    sell_or_csr((l == 0) ? h.A : L.A, 0);
+    // prolongator rows (1 to ~10 entries) are sorted by length within windows
+    // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted
+    if ((int)l != H.coarse) sell_or_csr(L.P, 1024);
   }
   HIP_CHECK(hipStreamSynchronize(s));
   H.built = true;

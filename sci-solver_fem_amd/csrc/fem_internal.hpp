@@ -48,6 +48,7 @@ struct Csr {
   DevBuf<int32_t> swin;   // kSellWindows bases per slice
   int64_t ncompact = 0;     // number of compact slices
   int64_t nnz_compact = 0;  // stored nonzeros in compact slices
+  DevBuf<int32_t> srow;     // row at each sliced position (empty: position = row)
   DevBuf<double> s64;   // padded FP64 values
   DevBuf<float> s32;    // padded FP32 values (mixed precision only)
 };
@@ -181,7 +182,7 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
 // (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when with_f32)
-void build_sell(Csr& A, bool with_f32, hipStream_t s);
+void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----

@@ -119,9 +119,12 @@ struct SellView {
   const int32_t* sc;    // int32 columns
   const uint16_t* sc16; // 16-bit column codes of compact slices
   const int32_t* swin;  // kSellWindows window bases per slice (-1: not compact)
+  const int32_t* srow;  // row at each position (null: position = row)
 };
 
-inline SellView sell_view(const Csr& A) { return SellView{A.sp.get(), A.sc.get(), A.sc16.get(), A.swin.get()}; }
+inline SellView sell_view(const Csr& A) {
+  return SellView{A.sp.get(), A.sc.get(), A.sc16.get(), A.swin.get(), A.srow.size() ? A.srow.get() : nullptr};
+}
 
 // Window bases of a compact slice, read once per wavefront into scalar
 // registers (named members, not an array: the select tree below must stay
@@ -196,8 +199,11 @@ __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, S
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
   const T acc = sell_row_sum<T>(S, slice, lane, sv, x);
-  const int64_t row = (int64_t)slice * 64 + lane;
-  if (row < n) spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
+  const int64_t pos = (int64_t)slice * 64 + lane;
+  if (pos < n) {
+    const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
+    spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
+  }
 }
 
 // ---- reductions ---------------------------------------------------------------

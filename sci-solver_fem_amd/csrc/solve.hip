@@ -33,12 +33,15 @@ int pow2_at_least(int k) {
   return p;
 }
 
+// lanes per row of the CSR kernels, from the average row length (measured at
+// 161^3: 64-entry coarse rows run fastest with 16 lanes, 100-entry
+// restriction rows with 32)
 int lanes_for(const Csr& A) {
   double avg = A.n ? (double)A.nnz / (double)A.n : 1.0;
   if (avg <= 6) return 4;
   if (avg <= 12) return 8;
-  if (avg <= 24) return 16;
-  if (avg <= 48) return 32;
+  if (avg <= 80) return 16;
+  if (avg <= 200) return 32;
   return 64;
 }
 
@@ -444,7 +447,8 @@ struct Level0Io {
 
 template <typename T>
 bool fused_out(const Level0Io* io, const Csr& A, int k) {
-  return std::is_same<T, float>::value && io && io->z64 && k == 1 && A.nslices > 0 && A.s32.size() > 0;
+  return std::is_same<T, float>::value && io && io->z64 && k == 1 && A.nslices > 0 && A.s32.size() > 0 &&
+         A.srow.size() == 0;
 }
 
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
@@ -801,6 +805,7 @@ void update(Handle& h, int k, double tol2) {
 
 void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
+  if (A.srow.size()) fail(FEM_ERR_STATE, "the operator's sliced-ELL copy must keep the row order");
   unsigned g = grid_for(A.nslices, kNT / 64);
   pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(),
                                                h.partials.get());

@@ -245,26 +245,46 @@ __global__ void transpose_ptr_kernel(int64_t m, const uint64_t* __restrict__ key
   rp[c] = (int32_t)lo;
 }
 
-__global__ void sell_width_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp, int32_t* __restrict__ w) {
+// row stored at sliced-ELL position p (rows may be reordered by length)
+__device__ __forceinline__ int64_t sell_row(const int32_t* __restrict__ srow, int64_t p) {
+  return srow ? (int64_t)srow[p] : p;
+}
+
+__global__ void sell_width_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+                                  const int32_t* __restrict__ srow, int32_t* __restrict__ w) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
   int mx = 0;
-  for (int64_t r = sl * 64; r < sl * 64 + 64 && r < n; ++r) mx = max(mx, rp[r + 1] - rp[r]);
+  for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n; ++p) {
+    const int64_t r = sell_row(srow, p);
+    mx = max(mx, rp[r + 1] - rp[r]);
+  }
   w[sl] = mx * 64;
+}
+
+// sort key of the length-ordered layout: window of `win` rows, then longest first
+__global__ void sell_sort_keys_kernel(int64_t n, const int32_t* __restrict__ rp, int64_t win,
+                                      uint32_t* __restrict__ key, int32_t* __restrict__ idx) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int len = min(rp[r + 1] - rp[r], 255);
+  key[r] = ((uint32_t)(r / win) << 8) | (uint32_t)(255 - len);
+  idx[r] = (int32_t)r;
 }
 
 // Window bases of each slice: the distinct aligned blocks (col >> kSellWinBits)
 // its columns touch, in order of first appearance (rows in order, entries in
 // row order).  More than kSellWindows blocks, or no entry at all: not compact.
 __global__ void sell_window_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
-                                   const int32_t* __restrict__ ci, int32_t* __restrict__ swin,
-                                   int32_t* __restrict__ compact) {
+                                   const int32_t* __restrict__ ci, const int32_t* __restrict__ srow,
+                                   int32_t* __restrict__ swin, int32_t* __restrict__ compact) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
   int blk[kSellWindows];
   int nb = 0;
   bool ok = true;
-  for (int64_t r = sl * 64; r < sl * 64 + 64 && r < n && ok; ++r) {
+  for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n && ok; ++p) {
+    const int64_t r = sell_row(srow, p);
     for (int q = rp[r]; q < rp[r + 1]; ++q) {
       const int b = ci[q] >> kSellWinBits;
       bool seen = false;
@@ -285,24 +305,30 @@ __global__ void sell_window_kernel(int64_t n, int64_t nslices, const int32_t* __
 
 // stored nonzeros of each compact slice (0 for the others)
 __global__ void sell_compact_nnz_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
-                                        const int32_t* __restrict__ compact, int64_t* __restrict__ out) {
+                                        const int32_t* __restrict__ srow, const int32_t* __restrict__ compact,
+                                        int64_t* __restrict__ out) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
-  const int64_t r0 = sl * 64, r1 = min(n, r0 + 64);
-  out[sl] = compact[sl] ? (int64_t)(rp[r1] - rp[r0]) : 0;
+  int64_t cnt = 0;
+  if (compact[sl])
+    for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n; ++p) {
+      const int64_t r = sell_row(srow, p);
+      cnt += rp[r + 1] - rp[r];
+    }
+  out[sl] = cnt;
 }
 
 // one thread per (slice, lane): row = slice * 64 + lane
 __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
                                  const int32_t* __restrict__ ci, const double* __restrict__ v,
-                                 const int32_t* __restrict__ sp, const int32_t* __restrict__ swin,
-                                 int32_t* __restrict__ sc, uint16_t* __restrict__ sc16, double* __restrict__ s64,
-                                 float* __restrict__ s32) {
+                                 const int32_t* __restrict__ srow, const int32_t* __restrict__ sp,
+                                 const int32_t* __restrict__ swin, int32_t* __restrict__ sc,
+                                 uint16_t* __restrict__ sc16, double* __restrict__ s64, float* __restrict__ s32) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslices * 64) return;
   int64_t sl = t >> 6;
   int lane = (int)(t & 63);
-  int64_t row = t;
+  int64_t row = t < n ? sell_row(srow, t) : t;
   int off = sp[sl];
   int w = (sp[sl + 1] - off) >> 6;
   int len = 0, base = 0;
@@ -409,13 +435,26 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
   HIP_LAUNCH_CHECK();
 }
 
-void build_sell(Csr& A, bool with_f32, hipStream_t s) {
+void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window) {
   const int64_t ns = (A.n + 63) / 64;
   A.nslices = ns;
   A.sp.resize(ns + 1);
+  A.srow.release();
+  if (sort_window > 0 && A.n > 0) {
+    // rows reordered, longest first, inside windows of sort_window rows: the
+    // slices of rows of mixed lengths (prolongators) then carry little padding
+    if ((A.n - 1) / sort_window >= ((int64_t)1 << 24)) fail(FEM_ERR_CAPACITY, "too many rows to sort by length");
+    DevBuf<uint32_t> key(A.n), key2(A.n);
+    DevBuf<int32_t> idx(A.n);
+    A.srow.resize(A.n);
+    sell_sort_keys_kernel<<<grid_for(A.n, 256), 256, 0, s>>>(A.n, A.rp.get(), sort_window, key.get(), idx.get());
+    HIP_LAUNCH_CHECK();
+    dev_sort_pairs_u32_i32(key.get(), key2.get(), idx.get(), A.srow.get(), (size_t)A.n, 32, s);
+  }
+  const int32_t* srow = A.srow.size() ? A.srow.get() : nullptr;
   DevBuf<int32_t> w(ns > 0 ? ns : 1);
   if (ns > 0) {
-    sell_width_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), w.get());
+    sell_width_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), srow, w.get());
     HIP_LAUNCH_CHECK();
   }
   int64_t tot = dev_exclusive_scan_i32(w.get(), A.sp.get(), ns, s);
@@ -429,16 +468,17 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s) {
   A.nnz_compact = 0;
   if (ns > 0) {
     DevBuf<int32_t> cflag(ns), cscan(ns + 1);
-    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.swin.get(), cflag.get());
+    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), srow, A.swin.get(),
+                                                         cflag.get());
     HIP_LAUNCH_CHECK();
     A.ncompact = dev_exclusive_scan_i32(cflag.get(), cscan.get(), ns, s);
     DevBuf<int64_t> snnz(ns), sscan(ns + 1);
-    sell_compact_nnz_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), cflag.get(), snnz.get());
+    sell_compact_nnz_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), srow, cflag.get(), snnz.get());
     HIP_LAUNCH_CHECK();
     A.nnz_compact = dev_exclusive_scan_i64(snnz.get(), sscan.get(), ns, s);
-    sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), A.sp.get(),
-                                                            A.swin.get(), A.sc.get(), A.sc16.get(), A.s64.get(),
-                                                            with_f32 ? A.s32.get() : nullptr);
+    sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), srow,
+                                                            A.sp.get(), A.swin.get(), A.sc.get(), A.sc16.get(),
+                                                            A.s64.get(), with_f32 ? A.s32.get() : nullptr);
     HIP_LAUNCH_CHECK();
   }
   HIP_CHECK(hipStreamSynchronize(s));
