@@ -105,8 +105,20 @@ __global__ void __launch_bounds__(kNT) csr_multi_kernel(int64_t n, int k, const 
   T acc = T(0);
   // plain loads: the KP lanes of a row walk its entries one by one, so each
   // cache line of val / ci is used over several iterations (a nontemporal
-  // hint would evict it after the first)
-  for (int q = s; q < e; ++q) acc += (T)val[q] * x[(int64_t)ci[q] * k + c];
+  // hint would evict it after the first).  Four entries per step keep four
+  // independent gathers in flight; the sum stays in entry order.
+  int q = s;
+  for (; q + 4 <= e; q += 4) {
+    const int c0 = ci[q], c1i = ci[q + 1], c2i = ci[q + 2], c3 = ci[q + 3];
+    const T v0 = (T)val[q], v1 = (T)val[q + 1], v2 = (T)val[q + 2], v3 = (T)val[q + 3];
+    const T g0 = x[(int64_t)c0 * k + c], g1 = x[(int64_t)c1i * k + c], g2 = x[(int64_t)c2i * k + c],
+            g3 = x[(int64_t)c3 * k + c];
+    acc += v0 * g0;
+    acc += v1 * g1;
+    acc += v2 * g2;
+    acc += v3 * g3;
+  }
+  for (; q < e; ++q) acc += (T)val[q] * x[(int64_t)ci[q] * k + c];
   spmv_epilogue<MODE, T, V>(row * k + c, row, acc, y, x, b, dinv, d, xn, c1, c2);
 }
 

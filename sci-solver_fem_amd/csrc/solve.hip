@@ -25,7 +25,11 @@ namespace fem {
 
 namespace {
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_ACT, S_RZN, S_TRUE, S_COUNT };
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_ACT, S_RZN, S_TRUE, S_SEQ, S_COUNT };
+
+// host-mapped record of one |r|^2 finish: act[64], rr[64], sequence number;
+// two records (by the parity of the sequence number)
+constexpr int kActRec = 136;
 
 int pow2_at_least(int k) {
   int p = 1;
@@ -292,7 +296,16 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
     for (int64_t row = r0 + threadIdx.x / KP; row < r1; row += per) {
       const int s = rp[row], e = rp[row + 1];
       double acc = 0.0;
-      for (int j = s; j < e; ++j) acc += val[j] * p[(int64_t)ci[j] * k + c];
+      int j = s;
+      for (; j + 4 <= e; j += 4) {
+        const double g0 = p[(int64_t)ci[j] * k + c], g1 = p[(int64_t)ci[j + 1] * k + c],
+                     g2 = p[(int64_t)ci[j + 2] * k + c], g3 = p[(int64_t)ci[j + 3] * k + c];
+        acc += val[j] * g0;
+        acc += val[j + 1] * g1;
+        acc += val[j + 2] * g2;
+        acc += val[j + 3] * g3;
+      }
+      for (; j < e; ++j) acc += val[j] * p[(int64_t)ci[j] * k + c];
       q[row * k + c] = acc;
       part += acc * p[row * k + c];
     }
@@ -347,6 +360,14 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
                                                           double* __restrict__ scal, int mode, int slot, double tol2,
                                                           double* __restrict__ host_act) {
   __shared__ double s_buf[kFinNT / 64];
+  __shared__ double s_seq;
+  // |r|^2 finishes are numbered on the device; record n goes to host record n % 2
+  if (mode == F_RR && threadIdx.x == 0) {
+    s_seq = scal[S_SEQ * k] + 1.0;
+    scal[S_SEQ * k] = s_seq;
+  }
+  __syncthreads();
+  double* rec = (mode == F_RR && host_act) ? host_act + ((int64_t)s_seq & 1) * kActRec : nullptr;
   for (int c = 0; c < k; ++c) {
     double v = 0.0;
 #pragma unroll 8
@@ -366,7 +387,10 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
         scal[S_RR * k + c] = v;
         bool act = scal[S_ACT * k + c] != 0.0 && v > tol2 * scal[S_BB * k + c];
         scal[S_ACT * k + c] = act ? 1.0 : 0.0;
-        if (host_act) host_act[c] = act ? 1.0 : 0.0;  // read by the host once the stream drains
+        if (rec) {
+          rec[c] = act ? 1.0 : 0.0;
+          rec[64 + c] = v;
+        }
       } else {
         double rz = scal[S_RZ * k + c];
         scal[S_BETA * k + c] = (scal[S_ACT * k + c] != 0.0 && rz != 0.0) ? v / rz : 0.0;
@@ -375,6 +399,7 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
     }
     __syncthreads();
   }
+  if (rec && threadIdx.x == 0) rec[128] = s_seq;  // read by the host after this kernel completed
 }
 
 // p = z + beta p
@@ -693,7 +718,8 @@ void ensure_krylov(Handle& h, int k) {
     h.scal.resize((size_t)S_COUNT * 64);
     if (!h.h_scal) HIP_CHECK(hipHostMalloc((void**)&h.h_scal, sizeof(double) * S_COUNT * 64, hipHostMallocDefault));
     if (!h.h_act) {
-      HIP_CHECK(hipHostMalloc((void**)&h.h_act, sizeof(double) * 64, hipHostMallocMapped | hipHostMallocCoherent));
+      HIP_CHECK(hipHostMalloc((void**)&h.h_act, sizeof(double) * 2 * kActRec,
+                              hipHostMallocMapped | hipHostMallocCoherent));
       HIP_CHECK(hipHostGetDevicePointer((void**)&h.d_act, h.h_act, 0));
     }
     h.kcap = k;
@@ -864,17 +890,21 @@ void set_scal(Handle& h, int slot, int k, const double* v) {
   HIP_CHECK(hipMemcpyAsync(h.scal.get() + slot * k, v, sizeof(double) * k, hipMemcpyHostToDevice, h.stream));
 }
 
-// Wait for the stream by polling (no sleep / wake-up between PCG iterations)
-// and read the convergence flags the last |r|^2 finish wrote to mapped host
-// memory: no copy is queued.
-void wait_act(Handle& h, int k, double* v) {
+// Convergence record `seq` of the |r|^2 finish (act flags and |r|^2 per
+// column), read from host-mapped memory once event `ev` (recorded after that
+// finish) has completed; polled, so there is no sleep / wake-up in between.
+void read_record(Handle& h, hipEvent_t ev, uint64_t seq, int k, double* act, double* rr) {
   for (;;) {
-    const hipError_t e = hipStreamQuery(h.stream);
+    const hipError_t e = hipEventQuery(ev);
     if (e == hipSuccess) break;
     if (e != hipErrorNotReady) HIP_CHECK(e);
   }
-  const volatile double* a = h.h_act;
-  for (int c = 0; c < k; ++c) v[c] = a[c];
+  const volatile double* rec = h.h_act + (seq & 1) * kActRec;
+  if (rec[128] != (double)seq) fail(FEM_ERR_STATE, "convergence record out of sequence");
+  for (int c = 0; c < k; ++c) {
+    act[c] = rec[c];
+    rr[c] = rec[64 + c];
+  }
 }
 
 void get_scal(Handle& h, int slot, int k, double* v) {
@@ -935,6 +965,20 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       any = any || act[c] != 0.0;
     }
     set_scal(h, S_ACT, k, act.data());
+    const std::vector<double> zero_seq(k, 0.0);
+    set_scal(h, S_SEQ, k, zero_seq.data());  // |r|^2 finishes of this solve are numbered from 1
+    uint64_t seq_issued = 0;
+    bool ahead = false;
+    hipEvent_t evs[2];
+    HIP_CHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    struct EvGuard {
+      hipEvent_t* e;
+      ~EvGuard() {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+      }
+    } ev_guard{evs};
     int restarts = 0;
     while (any) {
       // z = M r and rz = r.z (fused into the last level-0 smoothing step when possible)
@@ -955,20 +999,48 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
         spmv_dot(h, k);      // q = A p, alpha
         update(h, k, tol2);  // x, r, rr, act
       };
-      bool first = true;
-      while (iters < h.prm.maxIters) {
+      // Iteration n ends with the n-th |r|^2 finish of this solve; the host
+      // reads its record while iteration n + 1 may already run: the next
+      // iteration is queued ahead whenever the residual history predicts
+      // more than two further iterations.  An iteration queued after the
+      // last needed one leaves x and r unchanged (alpha = beta = 0 for
+      // converged columns).
+      const uint64_t seq0 = seq_issued;
+      auto issue = [&](bool first) {
         if (first) {
           spmv_dot(h, k);
           update(h, k, tol2);
-          first = false;
         } else {
           run_iteration(h, k, tol2, iteration);
         }
+        ++seq_issued;
+        HIP_CHECK(hipEventRecord(evs[seq_issued & 1], s));
+      };
+      if (iters < h.prm.maxIters) issue(true);
+      uint64_t seq_done = seq0;
+      std::vector<double> rr_prev(k, -1.0), rr_prev2(k, -1.0), rrn(k);
+      while (seq_done < seq_issued) {
+        const bool room = iters + (int)(seq_issued - seq_done) < h.prm.maxIters;
+        if (ahead && room && seq_issued == seq_done + 1) issue(false);
+        read_record(h, evs[(seq_done + 1) & 1], seq_done + 1, k, act.data(), rrn.data());
+        ++seq_done;
         ++iters;
-        wait_act(h, k, act.data());
         bool still = false;
-        for (int c = 0; c < k; ++c) still = still || act[c] != 0.0;
-        if (!still) break;
+        double remaining = 0.0;
+        for (int c = 0; c < k; ++c) {
+          if (act[c] == 0.0) continue;
+          still = true;
+          // iterations left at the geometric rate of the last two
+          double rem = 1e30;
+          if (rr_prev2[c] > 0.0 && rrn[c] > 0.0 && rrn[c] < rr_prev2[c])
+            rem = std::log(tol2 * bb[c] / rrn[c]) / (0.5 * std::log(rrn[c] / rr_prev2[c]));
+          remaining = std::max(remaining, rem);
+          rr_prev2[c] = rr_prev[c];
+          rr_prev[c] = rrn[c];
+        }
+        if (!still || iters >= h.prm.maxIters) break;
+        ahead = remaining > 2.0;
+        if (seq_issued == seq_done) issue(false);
       }
       // verify on the true residual; restart columns that drifted
       true_residual(rel);

@@ -205,3 +205,23 @@ def test_graph_replay_matches_eager(fem, gpu, oracle, meshgen):
     # tolerance 1e-11 on the device residual; the host re-evaluation of b - A x
     # carries ~|A||x| eps of rounding of its own (x ~ 1e4 here)
     assert np.linalg.norm(b - A @ runs[1][3][0]) / np.linalg.norm(b) <= 1e-10
+
+
+def test_queued_ahead_iteration_leaves_solution_unchanged(fem, gpu, oracle, meshgen):
+    """The next PCG iteration may be queued before the host has read the
+    previous one's convergence flags; an iteration queued after convergence
+    must not change x.  A solve capped at exactly the reported iteration count
+    (which cannot queue beyond the cap) gives bitwise the same x."""
+    report("[gpu] iteration queued past convergence")
+    xyz, conn, kappa, A, b = _problem(meshgen, oracle, 21, 2.0, 1e-2, seed=6)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+    base = dict(solverType=1, tolerance=1e-9, precision=1, smootherType=1, preInnerIters=2, postInnerIters=2,
+                topSize=64)
+    s.set_params(maxIters=500, **base)
+    x1, i1 = s.solve(b)
+    assert i1["status"] == 0
+    s.set_params(maxIters=i1["iterations"], **base)
+    x2, i2 = s.solve(b)
+    assert i2["status"] == 0 and i2["iterations"] == i1["iterations"]
+    assert np.array_equal(x1, x2)
+    s.close()
