@@ -218,6 +218,52 @@ __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, S
   }
 }
 
+// k right-hand sides with k a multiple of VW = 16 / sizeof(T): L lanes per
+// row, each lane owns VW consecutive columns and gathers them as one 16-byte
+// load (measured for k = 16 FP32 at 161^3: 1.6x the one-column-per-lane kernel).
+// Same per-column summation order as csr_multi_kernel.
+template <typename T, int L, int MODE>
+__global__ void __launch_bounds__(kNT) csr_multi_vec_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ ci, const T* __restrict__ val,
+                                                          const T* __restrict__ x, T* __restrict__ y,
+                                                          const T* __restrict__ b, const T* __restrict__ dinv,
+                                                          T* __restrict__ d, T* __restrict__ xn, T c1, T c2) {
+  constexpr int VW = 16 / (int)sizeof(T);
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  const int64_t row = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kNT + threadIdx.x) / L;
+  const int c0 = (int)(threadIdx.x & (L - 1)) * VW;
+  if (row >= n || c0 >= k) return;
+  const int s = rp[row], e = rp[row + 1];
+  vec acc = (vec)(T(0));
+  int q = s;
+  for (; q + 4 <= e; q += 4) {
+    const int j0 = ci[q], j1 = ci[q + 1], j2 = ci[q + 2], j3 = ci[q + 3];
+    const T v0 = val[q], v1 = val[q + 1], v2 = val[q + 2], v3 = val[q + 3];
+    const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+    const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+    acc += v0 * g0;
+    acc += v1 * g1;
+    acc += v2 * g2;
+    acc += v3 * g3;
+  }
+  for (; q < e; ++q) acc += val[q] * *(const vec*)(x + (int64_t)ci[q] * k + c0);
+  const int64_t idx = row * k + c0;
+  if (MODE == kAssign) {
+    *(vec*)(y + idx) = acc;
+  } else if (MODE == kResid) {
+    *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
+  } else if (MODE == kAdd) {
+    *(vec*)(y + idx) += acc;
+  } else if (MODE == kJacobi) {
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + (c1 * dinv[row]) * (*(const vec*)(b + idx) - acc);
+  } else {  // kCheb
+    const vec r = dinv[row] * (*(const vec*)(b + idx) - acc);
+    const vec dn = (c1 != T(0) ? c1 * *(const vec*)(d + idx) : (vec)(T(0))) + c2 * r;
+    *(vec*)(d + idx) = dn;
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + dn;
+  }
+}
+
 // ---- reductions ---------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -93,6 +93,14 @@ void launch_vec(const Csr& A, const V* val, const T* x, T* y, const T* b, const 
   csr_vec_kernel<V, T, G, MODE><<<grid, kNT, 0, s>>>(A.n, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn, c1, c2);
 }
 
+template <typename T, int MODE, int L>
+void launch_multi_vec(const Csr& A, int k, const T* val, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn,
+                      T c1, T c2, hipStream_t s) {
+  unsigned grid = grid_for(A.n * L, kNT);
+  csr_multi_vec_kernel<T, L, MODE><<<grid, kNT, 0, s>>>(A.n, k, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn,
+                                                        c1, c2);
+}
+
 template <typename V, typename T, int MODE, int KP>
 void launch_multi(const Csr& A, int k, const V* val, const T* x, T* y, const T* b, const V* dinv, T* d, T* xn, T c1,
                   T c2, hipStream_t s) {
@@ -126,6 +134,16 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
       case 16: launch_vec<T, T, MODE, 16>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       case 32: launch_vec<T, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       default: launch_vec<T, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+    }
+  } else if (k % (16 / (int)sizeof(T)) == 0) {
+    // 16-byte column groups (all multi-column vectors here come from hipMalloc)
+    switch (pow2_at_least(k / (16 / (int)sizeof(T)))) {
+      case 1: launch_multi_vec<T, MODE, 1>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 2: launch_multi_vec<T, MODE, 2>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_multi_vec<T, MODE, 4>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_multi_vec<T, MODE, 8>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_multi_vec<T, MODE, 16>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_multi_vec<T, MODE, 32>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
   } else {
     switch (pow2_at_least(k)) {

@@ -115,3 +115,63 @@ def test_c5_full_size_properties(fem, gpu, meshgen):
     report(f"      {i1['iterations']} iterations, {i1['num_levels']} levels, linearity error {lin:.2e}")
     assert lin <= 1e-6
     s.close()
+
+
+def test_c1_square65(fem, gpu, oracle, meshgen):
+    """C1: 65^2 square, P1 triangles, sine load, FP64 path against a direct
+    solve of the oracle's matrix, and the mixed path against the FP64 one."""
+    report("[gpu] full-size c1_square65_fp64")
+    c = meshgen.CONFIGS["c1_square65_fp64"]
+    xyz, tris = meshgen.square_mesh(c["N"], seed=0)
+    b = meshgen.sine_load_2d(xyz, tris)
+    A = oracle.scipy_matrix(xyz, tris, None, c["mass"])
+    s = fem.FEMSolver.from_arrays(2, xyz, tris, mass_coeff=c["mass"])
+    xs = {}
+    for prec in (0, 1):
+        s.set_params(precision=prec, **dict(PARAMS, tolerance=1e-10, topSize=64))
+        x, info = s.solve(b)
+        assert info["status"] == 0
+        xs[prec] = x
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(xs[0] - xd) / np.linalg.norm(xd) <= 1e-8
+    assert np.linalg.norm(xs[1] - xs[0]) / np.linalg.norm(xs[0]) <= 1e-8
+    s.close()
+
+
+def test_c4_adjoint_full_size(fem, gpu, meshgen):
+    """C4: 49^3, sigma 1, L = 0.5 |u - u_obs|^2 with u_obs from a second seeded
+    coefficient plus N(0, 1e-3) noise; the adjoint gradient against central
+    differences of L (GPU FP64 solves at 1e-12) along 4 seeded directions."""
+    report("[gpu] full-size c4_cube49_adjoint")
+    xyz, tets, kappa, mass = _config(meshgen, "c4_cube49_adjoint")
+    kappa2 = meshgen.lognormal_kappa(101, tets.shape[0], 1.0)
+    b = meshgen.gaussian_rhs(0, xyz.shape[0])
+    tight = dict(PARAMS, precision=0, tolerance=1e-12, maxIters=1000)
+    s2 = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa2, mass_coeff=mass)
+    s2.set_params(**tight)
+    u2, _ = s2.solve(b)
+    s2.close()
+    u_obs = u2 + meshgen.noise(7, xyz.shape[0], 1e-3)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    s.set_params(**tight)
+    u, info = s.solve(b)
+    g, _, ainfo = s.adjoint_grad(u, u - u_obs)
+    assert info["status"] == 0 and ainfo["status"] == 0
+
+    def loss(k):
+        s.update_kappa(k)
+        uk, ik = s.solve(b)
+        assert ik["status"] == 0
+        return 0.5 * float(np.dot(uk - u_obs, uk - u_obs))
+
+    rng = np.random.default_rng(49)
+    h = 1e-4
+    for _ in range(4):
+        d = rng.normal(size=kappa.size) * kappa
+        fd = (loss(kappa + h * d) - loss(kappa - h * d)) / (2 * h)
+        ad = float(np.dot(g, d))
+        err = abs(fd - ad) / (np.linalg.norm(g) * np.linalg.norm(d))
+        report(f"      FD {fd:.6e} adjoint {ad:.6e} scaled error {err:.1e}")
+        assert err < 1e-6
+    s.close()

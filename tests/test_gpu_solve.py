@@ -225,3 +225,27 @@ def test_queued_ahead_iteration_leaves_solution_unchanged(fem, gpu, oracle, mesh
     assert i2["status"] == 0 and i2["iterations"] == i1["iterations"]
     assert np.array_equal(x1, x2)
     s.close()
+
+
+@pytest.mark.parametrize("k", [3, 8, 16])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_widths(fem, gpu, oracle, meshgen, k, precision):
+    """k columns take the one-column-per-lane kernels (k = 3) or the 16-byte
+    column-group kernels (k a multiple of 4 floats / 2 doubles); every column
+    must match its single-column solve."""
+    report(f"[gpu] batched k={k} precision={precision}")
+    xyz, conn, kappa, A, _ = _problem(meshgen, oracle, 11, 1.5, 1e-2, seed=8)
+    s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-9, maxIters=300, precision=precision, preInnerIters=2,
+                 postInnerIters=2, topSize=64)
+    B = meshgen.gaussian_rhs(11, xyz.shape[0], k)
+    X, info = s.solve(B)
+    assert info["status"] == 0
+    Y = s.apply_operator(X)
+    # rounding of A X is relative to |A| |X| (X ~ 1e4 with A X ~ 1 here)
+    assert np.max(np.abs(Y - A @ X)) <= 1e-14 * np.max(abs(A) @ np.abs(X))
+    for c in range(k):
+        x1, _ = s.solve(np.ascontiguousarray(B[:, c]))
+        assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1e-9
+        assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) <= 1e-7
+    s.close()
