@@ -354,6 +354,51 @@ __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, doubl
   block_col_partials<KP>(part, k, s_buf, partials);
 }
 
+// The same for an even k: L lanes per row, two columns per lane as one
+// 16-byte load; per-column sums in the same order as pcg_spmm_dot_kernel.
+template <int L>
+__global__ void __launch_bounds__(kNT) pcg_spmm_dot_vec_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci,
+                                                             const double* __restrict__ val,
+                                                             const double* __restrict__ p, double* __restrict__ q,
+                                                             double* __restrict__ partials) {
+  typedef double vec __attribute__((ext_vector_type(2)));
+  __shared__ double s_buf[2 * kNT];
+  const int c0 = (int)(threadIdx.x & (L - 1)) * 2;
+  const int64_t per = kNT / L;
+  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
+  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
+  vec part = (vec)(0.0);
+  if (c0 < k) {
+    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += per) {
+      const int s = rp[row], e = rp[row + 1];
+      vec acc = (vec)(0.0);
+      int j = s;
+      for (; j + 4 <= e; j += 4) {
+        const vec g0 = *(const vec*)(p + (int64_t)ci[j] * k + c0), g1 = *(const vec*)(p + (int64_t)ci[j + 1] * k + c0);
+        const vec g2 = *(const vec*)(p + (int64_t)ci[j + 2] * k + c0),
+                  g3 = *(const vec*)(p + (int64_t)ci[j + 3] * k + c0);
+        acc += val[j] * g0;
+        acc += val[j + 1] * g1;
+        acc += val[j + 2] * g2;
+        acc += val[j + 3] * g3;
+      }
+      for (; j < e; ++j) acc += val[j] * *(const vec*)(p + (int64_t)ci[j] * k + c0);
+      *(vec*)(q + row * k + c0) = acc;
+      part += acc * *(const vec*)(p + row * k + c0);
+    }
+  }
+  s_buf[2 * threadIdx.x] = part.x;
+  s_buf[2 * threadIdx.x + 1] = part.y;
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    const int c = threadIdx.x;
+    double sum = 0.0;
+    for (int t = c / 2; t < kNT; t += L) sum += s_buf[2 * t + (c & 1)];
+    partials[(int64_t)blockIdx.x * k + c] = sum;
+  }
+}
+
 // partials of a.b per column
 template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const double* __restrict__ a,
@@ -873,8 +918,14 @@ void launch_spmm_dot(Handle& h, int k) {
   const Csr& A = h.A;
   int64_t g = (A.n * KP + kNT - 1) / kNT;
   if (g > 4096) g = 4096;
-  pcg_spmm_dot_kernel<KP><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, k, A.rp.get(), A.ci.get(), A.v64.get(),
-                                                             h.kp.get(), h.kq.get(), h.partials.get());
+  if (k % 2 == 0) {
+    // KP >= k, so KP / 2 lanes per row cover the k / 2 column pairs
+    pcg_spmm_dot_vec_kernel<(KP >= 2 ? KP / 2 : 1)><<<(unsigned)g, kNT, 0, h.stream>>>(
+        A.n, k, A.rp.get(), A.ci.get(), A.v64.get(), h.kp.get(), h.kq.get(), h.partials.get());
+  } else {
+    pcg_spmm_dot_kernel<KP><<<(unsigned)g, kNT, 0, h.stream>>>(A.n, k, A.rp.get(), A.ci.get(), A.v64.get(),
+                                                               h.kp.get(), h.kq.get(), h.partials.get());
+  }
   HIP_LAUNCH_CHECK();
   finish(h, (int)g, k, F_ALPHA, 0);
 }
