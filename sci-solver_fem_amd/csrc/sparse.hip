@@ -216,6 +216,96 @@ __global__ void __launch_bounds__(kNumBlock) num_kernel(int64_t nrows, const int
     for (int t = 0; t < len; ++t) c_v[o + t] = acc[t];
 }
 
+// Coarse-level products (few rows, thousands of products per row): a
+// wavefront per row.  Symbolic: the row's columns are marked in an LDS bitmap
+// of ncols bits (a set, so the marking order does not matter) and read back
+// in ascending order.  Numeric: the L entries of the row are taken in order;
+// for each, the lanes split the entries of the matching R row, which hit
+// distinct output columns, so every output value is summed in the same order
+// as in num_kernel and the result is bitwise the same.
+constexpr int kWaveBitmapCols = 1 << 17;  // 16 KB bitmap per row
+constexpr int kWaveAccCap = 2048;         // LDS accumulators per row (16 KB)
+
+template <bool kFill>
+__global__ void __launch_bounds__(64) sym_wave_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                    const int32_t* __restrict__ l_ci,
+                                                    const int32_t* __restrict__ r_rp,
+                                                    const int32_t* __restrict__ r_ci,
+                                                    const int32_t* __restrict__ colmap, int drop_diag, int64_t ncols,
+                                                    int32_t* __restrict__ cnt, const int32_t* __restrict__ c_rp,
+                                                    int32_t* __restrict__ c_ci) {
+  extern __shared__ unsigned bits[];
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int nw = (int)((ncols + 31) >> 5);
+  for (int w = lane; w < nw; w += 64) bits[w] = 0u;
+  __syncthreads();
+  for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
+    const int k = l_ci[p];
+    if (r_rp) {
+      for (int q = r_rp[k] + lane; q < r_rp[k + 1]; q += 64) {
+        const int c = colmap ? colmap[r_ci[q]] : r_ci[q];
+        if (!(drop_diag && c == (int)i)) atomicOr(&bits[c >> 5], 1u << (c & 31));
+      }
+    } else if (lane == 0) {
+      const int c = colmap ? colmap[k] : k;
+      if (!(drop_diag && c == (int)i)) atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+  }
+  __syncthreads();
+  // lane owns words [w0, w1); exclusive prefix of their popcounts gives its output offset
+  const int per = (nw + 63) / 64;
+  const int w0 = min(nw, lane * per), w1 = min(nw, w0 + per);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
+  int incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  if (!kFill) {
+    if (lane == 0) cnt[i] = total;
+    return;
+  }
+  int o = c_rp[i] + incl - mine;
+  for (int w = w0; w < w1; ++w) {
+    unsigned b = bits[w];
+    while (b) {
+      const int t = __ffs(b) - 1;
+      c_ci[o++] = (w << 5) + t;
+      b &= b - 1u;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) num_wave_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                    const int32_t* __restrict__ l_ci,
+                                                    const double* __restrict__ l_v,
+                                                    const int32_t* __restrict__ r_rp,
+                                                    const int32_t* __restrict__ r_ci,
+                                                    const double* __restrict__ r_v,
+                                                    const int32_t* __restrict__ c_rp,
+                                                    const int32_t* __restrict__ c_ci, double* __restrict__ c_v) {
+  __shared__ double s_acc[kWaveAccCap];
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int o = c_rp[i];
+  const int len = c_rp[i + 1] - o;
+  const int32_t* cols = c_ci + o;
+  double* acc = len <= kWaveAccCap ? s_acc : c_v + o;
+  for (int t = lane; t < len; t += 64) acc[t] = 0.0;
+  __syncthreads();
+  for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
+    const int k = l_ci[p];
+    const double a = l_v[p];
+    for (int q = r_rp[k] + lane; q < r_rp[k + 1]; q += 64) acc[bsearch_col(cols, len, r_ci[q])] += a * r_v[q];
+    __syncthreads();  // the next L entry may hit the same output columns
+  }
+  if (len <= kWaveAccCap)
+    for (int t = lane; t < len; t += 64) c_v[o + t] = acc[t];
+}
+
 __global__ void transpose_keys_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                       uint64_t* __restrict__ key, int32_t* __restrict__ idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -371,6 +461,9 @@ int bits_for(int64_t v) {
 
 }  // namespace
 
+// coarse products: few rows, output columns that fit an LDS bitmap
+bool use_wave_rows(int64_t nrows, int64_t ncols) { return nrows > 0 && nrows <= 65536 && ncols <= kWaveBitmapCols; }
+
 void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int64_t r_rows, const int32_t* colmap,
                      int64_t ncols, bool drop_diag, Csr& C, hipStream_t s) {
   (void)r_rows;
@@ -378,6 +471,22 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   C.n = n;
   C.m = ncols;
   C.rp.resize(n + 1);
+  if (use_wave_rows(n, ncols)) {
+    const size_t lds = sizeof(unsigned) * (size_t)((ncols + 31) / 32 > 0 ? (ncols + 31) / 32 : 1);
+    DevBuf<int32_t> cnt(n);
+    sym_wave_kernel<false><<<(unsigned)n, 64, lds, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
+                                                        drop_diag ? 1 : 0, ncols, cnt.get(), nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+    int64_t nnz = dev_exclusive_scan_i32(cnt.get(), C.rp.get(), n, s);
+    if (nnz >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "sparse product has 2^31 or more nonzeros");
+    C.nnz = nnz;
+    C.ci.resize(nnz > 0 ? nnz : 1);
+    sym_wave_kernel<true><<<(unsigned)n, 64, lds, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
+                                                       drop_diag ? 1 : 0, ncols, cnt.get(), C.rp.get(), C.ci.get());
+    HIP_LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
   if (n > 0) {
     sym_lds_kernel<false><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
@@ -429,6 +538,12 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
   C.v64.resize(C.nnz > 0 ? C.nnz : 1);
   if (C.n == 0) return;
+  if (use_wave_rows(C.n, C.m)) {
+    num_wave_kernel<<<(unsigned)C.n, 64, 0, s>>>(C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(), Rm.ci.get(),
+                                                  Rm.v64.get(), C.rp.get(), C.ci.get(), C.v64.get());
+    HIP_LAUNCH_CHECK();
+    return;
+  }
   num_kernel<<<grid_for(C.n, kNumBlock), kNumBlock, 0, s>>>(C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(),
                                                             Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(),
                                                             C.v64.get());

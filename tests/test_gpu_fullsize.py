@@ -14,8 +14,8 @@ from conftest import report
 pytestmark = pytest.mark.gpu
 
 # bench.py's V-cycle settings
-PARAMS = dict(solverType=1, tolerance=1e-8, maxIters=500, smootherType=1, preInnerIters=2, postInnerIters=2,
-              postRelaxes=1, topSize=256, aggregationRule=1)
+PARAMS = dict(solverType=1, tolerance=1e-8, maxIters=500, smootherType=1, preInnerIters=3, postInnerIters=3,
+              postRelaxes=1, topSize=256, aggregationRule=1, strengthThreshold=0.005)
 
 
 def _config(meshgen, name):
