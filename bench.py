@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--sigma", type=float, default=2.0)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-size", type=int, default=91, help="cube side of the CPU baseline sample")
-    ap.add_argument("--skip", default="", help="comma list of: fp64,batched,adjoint,cpu")
+    ap.add_argument("--skip", default="", help="comma list of: fp64,fp16,batched,adjoint,cpu")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -214,6 +214,30 @@ def main():
     prof = s.profile_stats("smooth_l0")
     pcg_prof = s.profile_stats("pcg_spmv")
     s.profile(False)
+
+    # the same solve with the level-0 smoother matrix in FP16 (precision 2;
+    # reported beside the FP32-hierarchy headline, not as it)
+    if "fp16" not in skip:
+        s.set_params(**solver_params(2))
+        t = time.perf_counter()
+        s.setup()
+        t16s = time.perf_counter() - t
+        dt16, x16, inf16, _ = time_solves(torch, s, b1, a.steps, 1)
+        s.profile(True)
+        s.solve(b1)
+        p16 = s.profile_stats("smooth_l0_h")
+        s.profile(False)
+        err16 = float(torch.linalg.norm(x16 - x_mixed) / torch.linalg.norm(x_mixed))
+        e = {"ms": dt16 * 1e3, "iterations": inf16[-1]["iterations"], "dof_per_s": n / dt16, "setup_s": t16s,
+             "rel_residual": inf16[-1]["rel_residual"], "vs_fp32_hierarchy_rel_diff": err16}
+        if p16["launches"]:
+            avg16 = p16["total_ms"] / p16["launches"]
+            e["smooth_step_us"] = round(avg16 * 1e3, 2)
+            e["smooth_step_GBs"] = round(p16["bytes_per_launch"] / (avg16 * 1e-3) / 1e9, 1)
+        out["mixed_fp16_smoother"] = e
+        log(f"[bench] fp16 level-0 smoother: {dt16 * 1e3:.1f} ms/solve, {e['iterations']} iterations")
+        s.set_params(**solver_params(1))
+        s.setup()
 
     # batched right-hand sides through one V-cycle (SpMM)
     if "batched" not in skip and a.nrhs > 1:

@@ -66,7 +66,11 @@ typedef struct fem_params {
   int device;             /* device_             0                            */
   int blockSize;          /* blockSize_          256 accepted, no effect      */
   /* --- additions --- */
-  int precision;          /* 0: FP64 hierarchy, 1: FP32 hierarchy + FP64 PCG  */
+  int precision;          /* 0: FP64 hierarchy, 1: FP32 hierarchy + FP64 PCG,
+                             2: as 1, and for one right-hand side the level-0
+                             smoother runs on D^-1/2 A D^-1/2 with FP16
+                             off-diagonals (row sums kept exact through the
+                             FP32 diagonal)                                   */
   int smootherType;       /* 0: Jacobi, weight smootherWeight*4/(3*rho(D^-1A))
                              1: Chebyshev of degree pre/postInnerIters        */
   int aggregationRule;    /* 0: first aggregated neighbour in index order (the

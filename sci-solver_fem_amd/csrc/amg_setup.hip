@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "fem_internal.hpp"
+#include "half.hpp"
 #include "kernels.hpp"
 
 namespace fem {
@@ -616,6 +617,99 @@ void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int 
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// ---- precision 2: level-0 smoother operator D^-1/2 A D^-1/2 with FP16 off-diagonals
+
+// off-diagonal part of row i, scaled: a_ij s_i s_j with s = 1/sqrt(diag) (the
+// product s_i s_j is formed first so that entries (i,j) and (j,i) are equal)
+__global__ void offdiag_scaled_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                      const double* __restrict__ v, const double* __restrict__ dinv,
+                                      int32_t* __restrict__ orp, int32_t* __restrict__ oci,
+                                      double* __restrict__ ov) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  orp[i] = rp[i] - (int32_t)i;  // every row holds its diagonal once
+  if (i == n) return;
+  const double si = sqrt(dinv[i]);
+  int o = rp[i] - (int32_t)i;
+  for (int q = rp[i]; q < rp[i + 1]; ++q) {
+    const int j = ci[q];
+    if (j == (int)i) continue;
+    oci[o] = j;
+    ov[o] = v[q] * (si * sqrt(dinv[j]));
+    ++o;
+  }
+}
+
+// diagonal 1 + eps_i that keeps the row sums of A:
+// eps_i = - sum_j (fp16(a^_ij) - a^_ij) sqrt(a_jj / a_ii); and s_i = 1/sqrt(a_ii)
+__global__ void half_diag_kernel(int64_t n, const int32_t* __restrict__ orp, const int32_t* __restrict__ oci,
+                                 const double* __restrict__ ov, const double* __restrict__ dinv,
+                                 float* __restrict__ diag, float* __restrict__ sv) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double si = sqrt(dinv[i]);
+  double eps = 0.0;
+  for (int q = orp[i]; q < orp[i + 1]; ++q) {
+    const double a = ov[q];
+    eps -= (half_value(a) - a) * (si / sqrt(dinv[oci[q]]));
+  }
+  diag[i] = (float)(1.0 + eps);
+  sv[i] = (float)si;
+}
+
+// rows of P times sqrt(a_ii)
+__global__ void scale_rows_sqrt_kernel(int64_t n, const int32_t* __restrict__ rp, const double* __restrict__ dinv,
+                                       double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = 1.0 / sqrt(dinv[i]);
+  for (int p = rp[i]; p < rp[i + 1]; ++p) v[p] *= q;
+}
+
+void copy_csr(const Csr& A, Csr& B, hipStream_t s) {
+  B.n = A.n;
+  B.m = A.m;
+  B.nnz = A.nnz;
+  B.rp.resize(A.n + 1);
+  B.ci.resize(A.nnz > 0 ? A.nnz : 1);
+  B.v64.resize(A.nnz > 0 ? A.nnz : 1);
+  HIP_CHECK(hipMemcpyAsync(B.rp.get(), A.rp.get(), sizeof(int32_t) * (A.n + 1), hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.ci.get(), A.ci.get(), sizeof(int32_t) * A.nnz, hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.v64.get(), A.v64.get(), sizeof(double) * A.nnz, hipMemcpyDeviceToDevice, s));
+}
+
+void build_half_level0(Handle& h) {
+  hipStream_t s = h.stream;
+  Hierarchy& H = h.H;
+  Level& L0 = *H.levels[0];
+  const Csr& A = h.A;
+  const int64_t n = A.n;
+  Csr& O = H.ah;
+  O.n = O.m = n;
+  O.nnz = A.nnz - n;
+  O.rp.resize(n + 1);
+  O.ci.resize(O.nnz > 0 ? O.nnz : 1);
+  O.v64.resize(O.nnz > 0 ? O.nnz : 1);
+  offdiag_scaled_kernel<<<grid_for(n + 1, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(),
+                                                              L0.dinv64.get(), O.rp.get(), O.ci.get(), O.v64.get());
+  HIP_LAUNCH_CHECK();
+  H.ah_diag.resize(n);
+  H.ah_s.resize(n);
+  half_diag_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, O.rp.get(), O.ci.get(), O.v64.get(), L0.dinv64.get(),
+                                                     H.ah_diag.get(), H.ah_s.get());
+  HIP_LAUNCH_CHECK();
+  build_sell(O, false, s, 0, /*with_f16=*/true, /*with_f64=*/false);
+  O.v64.release();  // only the FP16 sliced copy is used
+  // D^1/2 P (sliced ELL, FP32, rows sorted by length) and its transpose (CSR, FP32)
+  copy_csr(L0.P, H.ph, s);
+  scale_rows_sqrt_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, H.ph.rp.get(), L0.dinv64.get(), H.ph.v64.get());
+  HIP_LAUNCH_CHECK();
+  csr_transpose(H.ph, H.rh, s);
+  to_f32(H.ph, s);
+  to_f32(H.rh, s);
+  build_sell(H.ph, true, s, 1024);
+}
+
 void build_hierarchy(Handle& h) {
   auto t0 = std::chrono::steady_clock::now();
   ++h.gen;  // invalidates a captured PCG iteration
@@ -689,7 +783,7 @@ void build_hierarchy(Handle& h) {
         for (int q = rp[i]; q < rp[i + 1]; ++q) dense[i * n + ci[q]] += v[q];
       if (!dense_inverse(dense, n)) fail(FEM_ERR_NUMERIC, "coarsest-level matrix is singular");
       H.cinv64.upload(dense.data(), n * n, s);
-      if (H.precision == 1) {
+      if (H.precision >= 1) {
         H.cinv32.resize(n * n);
         dev_f64_to_f32(H.cinv64.get(), H.cinv32.get(), n * n, s);
       }
@@ -731,7 +825,7 @@ void build_hierarchy(Handle& h) {
   }
   H.grid_complexity = (double)total_rows / (double)h.A.n;
   H.operator_complexity = (double)total_nnz / (double)h.A.nnz;
-  if (H.precision == 1) {
+  if (H.precision >= 1) {
     for (size_t l = 0; l < H.levels.size(); ++l) {
       Level& L = *H.levels[l];
       Csr& A = (l == 0) ? h.A : L.A;
@@ -749,7 +843,7 @@ void build_hierarchy(Handle& h) {
   // which a lane per row serialises; they keep the multi-lane CSR kernels.
   auto sell_or_csr = [&](Csr& M, int64_t sort_window) {
     if (M.n > 0 && (double)M.nnz / (double)M.n <= 24.0) {
-      build_sell(M, H.precision == 1, s, sort_window);
+      build_sell(M, H.precision >= 1, s, sort_window);
     } else {
       M.nslices = 0;
       M.srow.release();
@@ -766,6 +860,9 @@ void build_hierarchy(Handle& h) {
     // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted
     if ((int)l != H.coarse) sell_or_csr(L.P, 1024);
   }
+  H.ah.nslices = 0;
+  H.ph.nslices = 0;
+  if (H.precision == 2 && H.coarse > 0) build_half_level0(h);
   HIP_CHECK(hipStreamSynchronize(s));
   H.built = true;
   H.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

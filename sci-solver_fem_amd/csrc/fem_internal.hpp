@@ -51,6 +51,7 @@ struct Csr {
   DevBuf<int32_t> srow;     // row at each sliced position (empty: position = row)
   DevBuf<double> s64;   // padded FP64 values
   DevBuf<float> s32;    // padded FP32 values (mixed precision only)
+  DevBuf<uint16_t> sh;  // padded FP16 values (bit patterns; precision 2, level 0 only)
 };
 
 struct Level {
@@ -78,6 +79,14 @@ struct Hierarchy {
   double setup_ms = 0;
   int precision = 0;
   bool built = false;
+  // precision 2: the level-0 smoother works on y = D^1/2 x with the operator
+  // D^-1/2 A D^-1/2 = diag(ah_diag) + (off-diagonals in FP16, `ah`); the
+  // rounding of each row's off-diagonals is moved into its diagonal so that
+  // the operator keeps the row sums of A.  ph = D^1/2 P, rh = ph^T.
+  Csr ah;
+  DevBuf<float> ah_diag;  // 1 + eps_i
+  DevBuf<float> ah_s;     // 1 / sqrt(a_ii)
+  Csr ph, rh;
 };
 
 struct KernelStat {
@@ -182,7 +191,8 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
 // (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when with_f32)
-void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0);
+void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0, bool with_f16 = false,
+                bool with_f64 = true);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----

@@ -152,15 +152,26 @@ __device__ __forceinline__ int sell_col(unsigned c, SellWin W) {
   return (s2 ? hi : lo) + (int)(c & ((1u << kSellWinBits) - 1u));
 }
 
+// value of a stored matrix entry: FP64 / FP32 as stored, FP16 kept as its
+// 16-bit pattern (uint16_t)
+template <typename T, typename V>
+__device__ __forceinline__ T sell_val(const V* p) {
+  return (T)ntload(p);
+}
+template <>
+__device__ __forceinline__ float sell_val<float, uint16_t>(const uint16_t* p) {
+  return (float)__builtin_bit_cast(_Float16, ntload(p));
+}
+
 // Row sum of slice `slice` (wave-uniform) for this lane's row.  Compact
 // slices read 2-byte column codes: bits 13-15 select one of the slice's 8
 // window bases, bits 0-12 are the offset.
-template <typename T>
-__device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane, const T* __restrict__ sv,
+template <typename T, typename V = T>
+__device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane, const V* __restrict__ sv,
                                           const T* __restrict__ x) {
   const int off = __builtin_amdgcn_readfirstlane(S.sp[slice]);
   const int w = __builtin_amdgcn_readfirstlane((S.sp[slice + 1] - off) >> 6);
-  const T* vp = sv + off + lane;
+  const V* vp = sv + off + lane;
   T acc = T(0);
   static_assert(kSellWindows == 8, "sell_col selects among 8 windows");
   const int32_t* wp = S.swin + slice * kSellWindows;
@@ -174,28 +185,28 @@ __device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane
     for (; k + 4 <= w; k += 4) {
       const unsigned c0 = ntload(cp + 64 * k), c1 = ntload(cp + 64 * (k + 1)), c2 = ntload(cp + 64 * (k + 2)),
                      c3 = ntload(cp + 64 * (k + 3));
-      const T v0 = ntload(vp + 64 * k), v1 = ntload(vp + 64 * (k + 1)), v2 = ntload(vp + 64 * (k + 2)),
-              v3 = ntload(vp + 64 * (k + 3));
+      const T v0 = sell_val<T>(vp + 64 * k), v1 = sell_val<T>(vp + 64 * (k + 1)),
+              v2 = sell_val<T>(vp + 64 * (k + 2)), v3 = sell_val<T>(vp + 64 * (k + 3));
       acc += v0 * x[sell_col(c0, W)];
       acc += v1 * x[sell_col(c1, W)];
       acc += v2 * x[sell_col(c2, W)];
       acc += v3 * x[sell_col(c3, W)];
     }
-    for (; k < w; ++k) acc += ntload(vp + 64 * k) * x[sell_col(ntload(cp + 64 * k), W)];
+    for (; k < w; ++k) acc += sell_val<T>(vp + 64 * k) * x[sell_col(ntload(cp + 64 * k), W)];
   } else {
     const int32_t* cp = S.sc + off + lane;
     int k = 0;
     for (; k + 4 <= w; k += 4) {
       const int c0 = ntload(cp + 64 * k), c1 = ntload(cp + 64 * (k + 1)), c2 = ntload(cp + 64 * (k + 2)),
                 c3 = ntload(cp + 64 * (k + 3));
-      const T v0 = ntload(vp + 64 * k), v1 = ntload(vp + 64 * (k + 1)), v2 = ntload(vp + 64 * (k + 2)),
-              v3 = ntload(vp + 64 * (k + 3));
+      const T v0 = sell_val<T>(vp + 64 * k), v1 = sell_val<T>(vp + 64 * (k + 1)),
+              v2 = sell_val<T>(vp + 64 * (k + 2)), v3 = sell_val<T>(vp + 64 * (k + 3));
       acc += v0 * x[c0];
       acc += v1 * x[c1];
       acc += v2 * x[c2];
       acc += v3 * x[c3];
     }
-    for (; k < w; ++k) acc += ntload(vp + 64 * k) * x[ntload(cp + 64 * k)];
+    for (; k < w; ++k) acc += sell_val<T>(vp + 64 * k) * x[ntload(cp + 64 * k)];
   }
   return acc;
 }

@@ -221,6 +221,85 @@ __global__ void __launch_bounds__(kNT) sell_smooth_out_kernel(int64_t n, int64_t
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// ---- precision 2: level-0 smoothing in y = D^1/2 x on D^-1/2 A D^-1/2 with
+// FP16 off-diagonals (diagonal 1 + eps_i held apart in FP32)
+
+// first step from y = 0: b^ = s fl32(r), y = c1 b^ (d = y if d != null)
+__global__ void scale_in_half_kernel(int64_t n, const float* __restrict__ sd, const double* __restrict__ r,
+                                     float* __restrict__ bh, float* __restrict__ y, float* __restrict__ d, float c1) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float bv = sd[i] * (float)r[i];
+  bh[i] = bv;
+  const float v = c1 * bv;
+  y[i] = v;
+  if (d) d[i] = v;
+}
+
+// r^ = b^ - (diag_i y_i + sum_j h_ij y_j); MODE kResid: out = r^;
+// kJacobi: yn = y + c1 r^; kCheb: d = c1 d + c2 r^, yn = y + d
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_half_kernel(int64_t n, int64_t nslices, SellView S,
+                                                      const uint16_t* __restrict__ sh, const float* __restrict__ diag,
+                                                      const float* __restrict__ y, const float* __restrict__ bh,
+                                                      float* __restrict__ out, float* __restrict__ d,
+                                                      float* __restrict__ yn, float c1, float c2) {
+  const int slice =
+      __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  if (slice >= nslices) return;
+  const float acc = sell_row_sum<float, uint16_t>(S, slice, lane, sh, y);
+  const int64_t row = (int64_t)slice * 64 + lane;
+  if (row >= n) return;
+  const float yi = y[row];
+  const float r = ntload(bh + row) - (ntload(diag + row) * yi + acc);
+  if (MODE == kResid) {
+    out[row] = r;
+  } else if (MODE == kJacobi) {
+    yn[row] = yi + c1 * r;
+  } else {
+    const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * r;
+    d[row] = dn;
+    yn[row] = yi + dn;
+  }
+}
+
+// last post-smoothing step: z = s y_new in FP64 and per-block partials of r.z
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_half_out_kernel(int64_t n, int64_t nslices, SellView S,
+                                                          const uint16_t* __restrict__ sh,
+                                                          const float* __restrict__ diag, const float* __restrict__ sd,
+                                                          const float* __restrict__ y, const float* __restrict__ bh,
+                                                          const float* __restrict__ d, float c1, float c2,
+                                                          const double* __restrict__ r64, double* __restrict__ z,
+                                                          double* __restrict__ partials) {
+  __shared__ double s_buf[kNT];
+  const int slice =
+      __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  if (slice < nslices) {
+    const float acc = sell_row_sum<float, uint16_t>(S, slice, lane, sh, y);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row < n) {
+      const float yi = y[row];
+      const float r = ntload(bh + row) - (ntload(diag + row) * yi + acc);
+      float yv;
+      if (MODE == kJacobi) {
+        yv = yi + c1 * r;
+      } else {
+        const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * r;
+        yv = yi + dn;
+      }
+      const double zv = (double)(sd[row] * yv);
+      z[row] = zv;
+      part = r64[row] * zv;
+    }
+  }
+  const double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
 template <typename Tin, typename Tout>
 __global__ void convert_kernel(int64_t m, const Tin* __restrict__ a, Tout* __restrict__ b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -670,6 +749,99 @@ T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr) {
   return cur;
 }
 
+// Chebyshev coefficients of step `step` of a smoother on [lmin, lmax]
+struct ChebSeq {
+  double theta, delta, sigma, rho_old;
+  explicit ChebSeq(const Smoother& S)
+      : theta(0.5 * (S.lmax + S.lmin)), delta(0.5 * (S.lmax - S.lmin)), sigma(theta / delta), rho_old(delta / theta) {}
+  void next(int step, double& c1, double& c2) {
+    if (step == 0) {
+      c1 = 0.0;
+      c2 = 1.0 / theta;
+    } else {
+      const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+      c1 = rho_new * rho_old;
+      c2 = 2.0 * rho_new / delta;
+      rho_old = rho_new;
+    }
+  }
+};
+
+// Precision 2, one column: the level-0 part of the V-cycle in y = D^1/2 x (the
+// same V-cycle as the FP32 one: Chebyshev/Jacobi on a matrix similar to
+// D^-1 A, restriction by (D^1/2 P)^T, prolongation by D^1/2 P), with the
+// FP64 residual in and z = D^-1/2 y out.  Returns the count of r.z partials.
+unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partials) {
+  Hierarchy& H = h.H;
+  Level& L0 = *H.levels[0];
+  hipStream_t s = h.stream;
+  const Csr& O = H.ah;
+  const int64_t n = O.n;
+  float* bh = L0.b32.get();
+  float* rr = L0.r32.get();
+  float* d = L0.d32.get();
+  float* cur = L0.x32.get();
+  float* nxt = L0.t32.get();
+  const Smoother S = smoother_for(h, L0);
+  const int nu_pre = h.prm.preInnerIters;
+  const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
+  const unsigned g = grid_for(O.nslices, kNT / 64);
+  const SellView V = sell_view(O);
+  // algorithmic bytes: FP16 value + column code per off-diagonal, and y, b^, diag (+ d r/w), y_new
+  const double mbytes = 2.0 * (double)O.nnz + 2.0 * (double)O.nnz_compact + 4.0 * ((double)O.nnz - O.nnz_compact);
+  const double step_bytes = mbytes + 4.0 * n * (S.type ? 6.0 : 4.0);
+  const bool prof = h.profiling;
+  auto step = [&](int mode, double c1, double c2) {
+    ProfScope ps(h, prof ? "smooth_l0_h" : "", step_bytes);
+    if (mode == kJacobi)
+      sell_half_kernel<kJacobi><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, nullptr,
+                                                  nullptr, nxt, (float)c1, 0.0f);
+    else
+      sell_half_kernel<kCheb><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, nullptr, d,
+                                                nxt, (float)c1, (float)c2);
+    HIP_LAUNCH_CHECK();
+    std::swap(cur, nxt);
+  };
+  // pre-smoothing from y = 0
+  ChebSeq cs(S);
+  scale_in_half_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, H.ah_s.get(), r64, bh, cur, S.type ? d : nullptr,
+                                                         (float)(S.type ? 1.0 / cs.theta : S.omega));
+  HIP_LAUNCH_CHECK();
+  for (int st = 1; st < nu_pre; ++st) {
+    double c1 = S.omega, c2 = 0.0;
+    if (S.type) cs.next(st, c1, c2);
+    step(S.type ? kCheb : kJacobi, c1, c2);
+  }
+  // residual, restriction, coarse correction, prolongation
+  sell_half_kernel<kResid><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, rr, nullptr,
+                                              nullptr, 0.0f, 0.0f);
+  HIP_LAUNCH_CHECK();
+  Level& N = *H.levels[1];
+  float* bc = N.b32.get();
+  spmv_op<float, kAssign>(H.rh, 1, rr, bc, nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
+  float* xc = vcycle<float>(h, 1, bc, 1);
+  spmv_op<float, kAdd>(H.ph, 1, xc, cur, nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
+  // post-smoothing; the last step writes z and the r.z partials
+  ChebSeq cp(S);
+  for (int st = 0; st < nu_post; ++st) {
+    double c1 = S.omega, c2 = 0.0;
+    if (S.type) cp.next(st, c1, c2);
+    if (st < nu_post - 1) {
+      step(S.type ? kCheb : kJacobi, c1, c2);
+      continue;
+    }
+    ProfScope ps(h, prof ? "smooth_l0_h_out" : "", step_bytes + 12.0 * n);
+    if (S.type)
+      sell_half_out_kernel<kCheb><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), H.ah_s.get(), cur,
+                                                    bh, d, (float)c1, (float)c2, r64, z64, partials);
+    else
+      sell_half_out_kernel<kJacobi><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), H.ah_s.get(),
+                                                      cur, bh, nullptr, (float)c1, 0.0f, r64, z64, partials);
+    HIP_LAUNCH_CHECK();
+  }
+  return g;
+}
+
 // z = M r at level 0 (FP64 in and out); k columns.  With `partials` (one
 // column, mixed precision, level 0 smoothed before and after the coarse
 // correction, sliced ELL) the conversions are fused into the first and last
@@ -688,6 +860,7 @@ unsigned precondition(Handle& h, const double* r, double* z, int k, double* part
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
   const bool fuse = partials && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
                     h.A.nslices > 0 && h.A.s32.size() > 0;
+  if (fuse && h.H.precision == 2 && h.H.ah.nslices > 0) return vcycle0_half(h, r, z, partials);
   if (fuse) {
     Level0Io io;
     io.r64 = r;
@@ -801,7 +974,9 @@ void ensure_krylov(Handle& h, int k) {
   }
 }
 
-void apply_vcycle_fp64(Handle& h, const double* r, double* z) { precondition(h, r, z, 1); }
+// the preconditioner as PCG applies it (for one column: the fused level-0 ends
+// and, in precision 2, the FP16 level-0 smoother; the r.z partials are unused)
+void apply_vcycle_fp64(Handle& h, const double* r, double* z) { precondition(h, r, z, 1, h.partials.get()); }
 
 namespace {
 

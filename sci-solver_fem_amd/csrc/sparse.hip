@@ -13,6 +13,7 @@
 // finding each column by binary search.  No float atomics: results are
 // bit-reproducible.
 #include "fem_internal.hpp"
+#include "half.hpp"
 
 #include <algorithm>
 
@@ -413,7 +414,8 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
                                  const int32_t* __restrict__ ci, const double* __restrict__ v,
                                  const int32_t* __restrict__ srow, const int32_t* __restrict__ sp,
                                  const int32_t* __restrict__ swin, int32_t* __restrict__ sc,
-                                 uint16_t* __restrict__ sc16, double* __restrict__ s64, float* __restrict__ s32) {
+                                 uint16_t* __restrict__ sc16, double* __restrict__ s64, float* __restrict__ s32,
+                                 uint16_t* __restrict__ sh) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslices * 64) return;
   int64_t sl = t >> 6;
@@ -448,8 +450,9 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
       sc16[e] = 0;
     }
     const double val = k < len ? v[base + k] : 0.0;
-    s64[e] = val;
+    if (s64) s64[e] = val;
     if (s32) s32[e] = (float)val;
+    if (sh) sh[e] = half_bits(val);
   }
 }
 
@@ -550,7 +553,7 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
   HIP_LAUNCH_CHECK();
 }
 
-void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window) {
+void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool with_f16, bool with_f64) {
   const int64_t ns = (A.n + 63) / 64;
   A.nslices = ns;
   A.sp.resize(ns + 1);
@@ -577,8 +580,9 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window) {
   A.sc.resize(tot > 0 ? tot : 1);
   A.sc16.resize(tot > 0 ? tot : 1);
   A.swin.resize(ns > 0 ? ns * kSellWindows : kSellWindows);
-  A.s64.resize(tot > 0 ? tot : 1);
+  if (with_f64) A.s64.resize(tot > 0 ? tot : 1); else A.s64.release();
   if (with_f32) A.s32.resize(tot > 0 ? tot : 1); else A.s32.release();
+  if (with_f16) A.sh.resize(tot > 0 ? tot : 1); else A.sh.release();
   A.ncompact = 0;
   A.nnz_compact = 0;
   if (ns > 0) {
@@ -593,7 +597,9 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window) {
     A.nnz_compact = dev_exclusive_scan_i64(snnz.get(), sscan.get(), ns, s);
     sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), srow,
                                                             A.sp.get(), A.swin.get(), A.sc.get(), A.sc16.get(),
-                                                            A.s64.get(), with_f32 ? A.s32.get() : nullptr);
+                                                            with_f64 ? A.s64.get() : nullptr,
+                                                            with_f32 ? A.s32.get() : nullptr,
+                                                            with_f16 ? A.sh.get() : nullptr);
     HIP_LAUNCH_CHECK();
   }
   HIP_CHECK(hipStreamSynchronize(s));
