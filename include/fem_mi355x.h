@@ -201,6 +201,14 @@ int fem_get_level_matrix(const fem_handle* h, int lvl, int32_t* row_ptr, int32_t
 /* Prolongator of level lvl -> lvl+1 (n_lvl x n_lvl+1 CSR); NULL skips. */
 int fem_get_level_prolongator(const fem_handle* h, int lvl, int64_t* nnz,
                               int32_t* row_ptr, int32_t* cols, double* vals);
+/* y = M x for level lvl's operator (which = 0, n_lvl x n_lvl), prolongator
+ * (1, n_lvl x n_lvl+1) or restriction (2, n_lvl+1 x n_lvl) through the
+ * one-column kernels the V-cycle launches, in the precision it applies them
+ * (FP32 values and vectors in precisions 1 and 2, FP64 in 0); host arrays in
+ * internal numbering.  No reference counterpart: a test and diagnostic hook
+ * beside the level exports above (the reference applies these matrices inside
+ * AMG_Level::cycle, src/core/cuda/amg_level.cu:22-128). */
+int fem_apply_level(fem_handle* h, int lvl, int which, const double* x, double* y);
 /* internal -> user vertex map (nv entries). */
 int fem_get_permutation(const fem_handle* h, int32_t* perm);
 

@@ -334,6 +334,22 @@ class FEMSolver:
                                                         v.ctypes.data))
         return rp, ci, v
 
+    def apply_level(self, lvl: int, which: int, x: np.ndarray) -> np.ndarray:
+        """y = M x for level `lvl`'s operator (which 0), prolongator (1) or
+        restriction (2), through the V-cycle's one-column kernels in the
+        hierarchy's precision (internal numbering)."""
+        n, nc = C.c_int64(), C.c_int64()
+        _check(self._h, lib().fem_get_level_info(self._h, lvl, C.byref(n), None))
+        if which in (1, 2):
+            _check(self._h, lib().fem_get_level_info(self._h, lvl + 1, C.byref(nc), None))
+        rows, cols = {0: (n.value, n.value), 1: (n.value, nc.value), 2: (nc.value, n.value)}.get(which, (0, 0))
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        if xx.shape != (cols,):
+            raise ValueError(f"x must have {cols} entries")
+        y = np.empty(rows, dtype=np.float64)
+        _check(self._h, lib().fem_apply_level(self._h, lvl, which, xx.ctypes.data, y.ctypes.data))
+        return y
+
     def permutation(self) -> np.ndarray:
         p = np.empty(self.getMatrixRows(), dtype=np.int32)
         _check(self._h, lib().fem_get_permutation(self._h, p.ctypes.data))

@@ -678,6 +678,44 @@ void copy_csr(const Csr& A, Csr& B, hipStream_t s) {
   HIP_CHECK(hipMemcpyAsync(B.v64.get(), A.v64.get(), sizeof(double) * A.nnz, hipMemcpyDeviceToDevice, s));
 }
 
+namespace {
+
+// Lanes per row of a sliced-ELL copy, from the average row length: one lane
+// up to 24 entries (fine operators, prolongators), else the fewest lanes that
+// leave at most 20 entries to each (161^3: ~80-entry level-1 rows take 4
+// lanes, 18.4 us per Chebyshev step against 34.6 us for the 16-lane CSR
+// kernel; ~140-entry level-0 restriction rows take 8, 37.4 us against 62.4 us;
+// 2x lanes, or rows sorted by length, measured slower).  0 = no sliced copy:
+// operators under 32768 rows with long rows have too few slices to fill the
+// chip and keep the CSR kernel (4,904 rows: 7.2 us sliced, 4.8 us CSR).
+int sell_lanes(const Csr& M) {
+  if (M.n == 0) return 0;
+  const double avg = (double)M.nnz / (double)M.n;
+  if (avg <= 24.0) return 1;
+  if (M.n < 32768) return 0;
+  int g = 2;
+  while (g < 32 && avg > 20.0 * g) g *= 2;
+  return g;
+}
+
+void sell_or_csr(Csr& M, int64_t sort_window, int lanes, bool with_f32, hipStream_t s) {
+  if (lanes > 0) {
+    build_sell(M, with_f32, s, sort_window, false, true, lanes);
+    return;
+  }
+  M.nslices = 0;
+  M.sell_g = 1;
+  M.srow.release();
+  M.sp.release();
+  M.sc.release();
+  M.sc16.release();
+  M.swin.release();
+  M.s64.release();
+  M.s32.release();
+}
+
+}  // namespace
+
 void build_half_level0(Handle& h) {
   hipStream_t s = h.stream;
   Hierarchy& H = h.H;
@@ -708,6 +746,7 @@ void build_half_level0(Handle& h) {
   to_f32(H.ph, s);
   to_f32(H.rh, s);
   build_sell(H.ph, true, s, 1024);
+  sell_or_csr(H.rh, 0, sell_lanes(H.rh), true, s);
 }
 
 void build_hierarchy(Handle& h) {
@@ -838,30 +877,25 @@ void build_hierarchy(Handle& h) {
       }
     }
   }
-  // Sliced-ELL copies for the solve phase, for operators with short rows (the
-  // fine level, prolongators).  Coarse operators have long rows (~50-300) for
-  // which a lane per row serialises; they keep the multi-lane CSR kernels.
-  auto sell_or_csr = [&](Csr& M, int64_t sort_window) {
-    if (M.n > 0 && (double)M.nnz / (double)M.n <= 24.0) {
-      build_sell(M, H.precision >= 1, s, sort_window);
-    } else {
-      M.nslices = 0;
-      M.srow.release();
-      M.sp.release();
-      M.sc.release();
-      M.s64.release();
-      M.s32.release();
-    }
-  };
+  // Sliced-ELL copies for the solve phase, built for every operator, restriction and
+  // prolongator the one-column solve applies, except small coarse operators with long rows
+  // (see sell_lanes).
   for (size_t l = 0; l < H.levels.size(); ++l) {
     Level& L = *H.levels[l];
-    sell_or_csr((l == 0) ? h.A : L.A, 0);
-    // prolongator rows (1 to ~10 entries) are sorted by length within windows
-    // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted
-    if ((int)l != H.coarse) sell_or_csr(L.P, 1024);
+    Csr& A = (l == 0) ? h.A : L.A;
+    // operators stay in row order (the fused PCG and level-0 smoother kernels
+    // read the fine one with one lane per row, in row order)
+    sell_or_csr(A, 0, sell_lanes(A), H.precision >= 1, s);
+    if ((int)l != H.coarse) {
+      // prolongator rows (1 to ~10 entries) are sorted by length within windows
+      // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted
+      sell_or_csr(L.P, 1024, sell_lanes(L.P), H.precision >= 1, s);
+      sell_or_csr(L.R, 0, sell_lanes(L.R), H.precision >= 1, s);
+    }
   }
   H.ah.nslices = 0;
   H.ph.nslices = 0;
+  H.rh.nslices = 0;
   if (H.precision == 2 && H.coarse > 0) build_half_level0(h);
   HIP_CHECK(hipStreamSynchronize(s));
   H.built = true;

@@ -523,6 +523,21 @@ int fem_apply_preconditioner(fem_handle* hh, const double* r, double* z, int on_
   });
 }
 
+int fem_apply_level(fem_handle* hh, int lvl, int which, const double* x, double* y) {
+  if (!hh || !x || !y) return FEM_ERR_ARG;
+  return guarded(hh, [&]() {
+    Handle& h = *H(hh);
+    ensure_setup(h);
+    const Csr& M = level_matrix_of(h, lvl, which);
+    DevBuf<double> dx(M.m > 0 ? M.m : 1), dy(M.n > 0 ? M.n : 1);
+    dx.upload(x, M.m, h.stream);
+    apply_level_int(h, lvl, which, dx.get(), dy.get());
+    dy.download(y, M.n, h.stream);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    return (int)FEM_OK;
+  });
+}
+
 int fem_get_num_levels(const fem_handle* h) { return h ? (int)H(h)->H.levels.size() : 0; }
 
 int fem_get_level_info(const fem_handle* hc, int lvl, int64_t* n, int64_t* nnz) {

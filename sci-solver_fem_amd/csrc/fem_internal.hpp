@@ -33,15 +33,18 @@ struct Csr {
   DevBuf<int32_t> ci;  // nnz
   DevBuf<double> v64;  // nnz
   DevBuf<float> v32;   // nnz (mixed precision only)
-  // Sliced ELL copy for the solve phase: slices of 64 consecutive rows (one
-  // wavefront, a lane per row), stored column-major inside the slice and
-  // padded to the longest row of the slice (padding: value 0, column = the row's last column).
+  // Sliced ELL copy for the solve phase: slices of 64 / sell_g consecutive
+  // rows (one wavefront, sell_g lanes per row), stored column-major inside the
+  // slice: entry j of the slice's row q sits at step j / sell_g of lane
+  // q * sell_g + j % sell_g.  Slices are padded to their longest row (padding:
+  // value 0, column = the row's last column).
   // Columns are kept twice: as int32 (sc) and as 16-bit codes (sc16) for the
   // compact slices, those whose columns fall in at most kSellWindows aligned
   // windows of 2^kSellWinBits columns.  swin holds each slice's window bases
   // (column offsets, multiples of 2^kSellWinBits; first entry -1 for a slice
   // that is not compact); a code is (window << kSellWinBits) | (col - base).
   int64_t nslices = 0;
+  int sell_g = 1;       // lanes per row (1, 2, 4, 8, 16 or 32)
   DevBuf<int32_t> sp;   // nslices + 1 element offsets (multiples of 64)
   DevBuf<int32_t> sc;   // padded columns
   DevBuf<uint16_t> sc16;  // padded 16-bit column codes (compact slices)
@@ -190,9 +193,10 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 // values of C = L * Rm (FP64) for the pattern found by spgemm_symbolic
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
-// (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when with_f32)
+// (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when
+// with_f32) with `lanes` (1, 2, 4, 8, 16 or 32) lanes per row
 void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0, bool with_f16 = false,
-                bool with_f64 = true);
+                bool with_f64 = true, int lanes = 1);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----
@@ -206,6 +210,11 @@ void ensure_krylov(Handle& h, int k);
 int pcg_solve(Handle& h, int k, fem_solve_info* info);   // uses h.kb (rhs), h.kx (x0 / out), internal order
 void apply_vcycle_fp64(Handle& h, const double* r, double* z);  // level-0 preconditioner, 1 column
 void apply_operator_int(Handle& h, const double* x, double* y, int k);
+// level lvl's operator (which 0), prolongator (1) or restriction (2)
+const Csr& level_matrix_of(const Handle& h, int lvl, int which);
+// y = M x for that matrix through the one-column solve kernels, in the
+// precision the V-cycle applies it (device FP64 arrays in and out)
+void apply_level_int(Handle& h, int lvl, int which, const double* x, double* y);
 
 // Profiling: when enabled, HIP events bracket each launch of a kernel family
 // on the handle's stream; they are read back (without stalling the stream

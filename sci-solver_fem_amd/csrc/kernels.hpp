@@ -212,7 +212,9 @@ __device__ __forceinline__ T sell_row_sum(const SellView& S, int slice, int lane
   return acc;
 }
 
-template <typename T, int MODE>
+// G lanes per row (slices of 64 / G rows): each lane sums its strided share
+// of the row, then a butterfly over the G lanes adds the shares (fixed order).
+template <typename T, int MODE, int G = 1>
 __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, SellView S, const T* __restrict__ sv,
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  const T* __restrict__ b, const T* __restrict__ dinv,
@@ -222,9 +224,11 @@ __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, S
       __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
-  const T acc = sell_row_sum<T>(S, slice, lane, sv, x);
-  const int64_t pos = (int64_t)slice * 64 + lane;
-  if (pos < n) {
+  T acc = sell_row_sum<T>(S, slice, lane, sv, x);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += shfl_xor_t(acc, off, G);
+  const int64_t pos = (int64_t)slice * (64 / G) + lane / G;
+  if ((G == 1 || (lane & (G - 1)) == 0) && pos < n) {
     const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
     spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
   }

@@ -116,6 +116,12 @@ const double* svals_of<double>(const Csr& A) { return A.s64.get(); }
 template <>
 const float* svals_of<float>(const Csr& A) { return A.s32.get(); }
 
+template <typename T, int MODE, int G>
+void launch_sell(const Csr& A, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn, T c1, T c2, hipStream_t s) {
+  sell_kernel<T, MODE, G><<<grid_for(A.nslices, kNT / 64), kNT, 0, s>>>(A.n, A.nslices, sell_view(A), svals_of<T>(A), x,
+                                                                       y, b, dinv, d, xn, c1, c2);
+}
+
 // y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.  One RHS
 // uses the sliced-ELL copy when it exists, otherwise CSR with G lanes per row.
 template <typename T, int MODE>
@@ -124,9 +130,14 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
   if (A.n == 0) return;
   const T* val = vals_of<T>(A);
   if (k == 1 && A.nslices > 0 && svals_of<T>(A)) {
-    unsigned grid = grid_for(A.nslices, kNT / 64);
-    sell_kernel<T, MODE><<<grid, kNT, 0, s>>>(A.n, A.nslices, sell_view(A), svals_of<T>(A), x, y, b, dinv, d, xn,
-                                              c1, c2);
+    switch (A.sell_g) {
+      case 1: launch_sell<T, MODE, 1>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 2: launch_sell<T, MODE, 2>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_sell<T, MODE, 4>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_sell<T, MODE, 8>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_sell<T, MODE, 16>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_sell<T, MODE, 32>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
+    }
   } else if (k == 1) {
     switch (lanes_for(A)) {
       case 4: launch_vec<T, T, MODE, 4>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
@@ -615,7 +626,7 @@ struct Level0Io {
 template <typename T>
 bool fused_out(const Level0Io* io, const Csr& A, int k) {
   return std::is_same<T, float>::value && io && io->z64 && k == 1 && A.nslices > 0 && A.s32.size() > 0 &&
-         A.srow.size() == 0;
+         A.srow.size() == 0 && A.sell_g == 1;
 }
 
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
@@ -859,7 +870,7 @@ unsigned precondition(Handle& h, const double* r, double* z, int k, double* part
   }
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
   const bool fuse = partials && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
-                    h.A.nslices > 0 && h.A.s32.size() > 0;
+                    h.A.nslices > 0 && h.A.s32.size() > 0 && h.A.sell_g == 1;
   if (fuse && h.H.precision == 2 && h.H.ah.nslices > 0) return vcycle0_half(h, r, z, partials);
   if (fuse) {
     Level0Io io;
@@ -941,6 +952,36 @@ void spmv64(const Csr& A, const double* x, double* y, hipStream_t s) {
 
 void apply_operator_int(Handle& h, const double* x, double* y, int k) {
   spmv_op<double, kAssign>(h.A, k, x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, h.stream);
+}
+
+const Csr& level_matrix_of(const Handle& h, int lvl, int which) {
+  const Hierarchy& H = h.H;
+  if (lvl < 0 || lvl >= (int)H.levels.size()) fail(FEM_ERR_ARG, "level out of range");
+  if (which == 0) return level_A(h, lvl);
+  if (which != 1 && which != 2) fail(FEM_ERR_ARG, "which: 0 operator, 1 prolongator, 2 restriction");
+  if (lvl == H.coarse) fail(FEM_ERR_ARG, "the coarsest level has no prolongator or restriction");
+  return which == 1 ? H.levels[lvl]->P : H.levels[lvl]->R;
+}
+
+void apply_level_int(Handle& h, int lvl, int which, const double* x, double* y) {
+  const Csr& M = level_matrix_of(h, lvl, which);
+  hipStream_t s = h.stream;
+  if (h.H.precision == 0) {
+    spmv_op<double, kAssign>(M, 1, x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
+  } else {
+    if (!M.v32.size() || (M.nslices > 0 && !M.s32.size())) fail(FEM_ERR_STATE, "no FP32 copy of this matrix");
+    DevBuf<float> xf(M.m > 0 ? M.m : 1), yf(M.n > 0 ? M.n : 1);
+    if (M.m > 0) {
+      convert_kernel<double, float><<<grid_for(M.m, 256), 256, 0, s>>>(M.m, x, xf.get());
+      HIP_LAUNCH_CHECK();
+    }
+    spmv_op<float, kAssign>(M, 1, xf.get(), yf.get(), nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
+    if (M.n > 0) {
+      convert_kernel<float, double><<<grid_for(M.n, 256), 256, 0, s>>>(M.n, yf.get(), y);
+      HIP_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
 }
 
 void ensure_krylov(Handle& h, int k) {
@@ -1069,7 +1110,8 @@ void update(Handle& h, int k, double tol2) {
 
 void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
-  if (A.srow.size()) fail(FEM_ERR_STATE, "the operator's sliced-ELL copy must keep the row order");
+  if (A.srow.size() || A.sell_g != 1)
+    fail(FEM_ERR_STATE, "the operator's sliced-ELL copy must keep the row order with one lane per row");
   unsigned g = grid_for(A.nslices, kNT / 64);
   pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(),
                                                h.partials.get());
@@ -1108,7 +1150,7 @@ void launch_spmm_dot(Handle& h, int k) {
 void spmv_dot(Handle& h, int k) {
   const Csr& A = h.A;
   ProfScope ps(h, "pcg_spmv", matrix_bytes<double>(A, k) + 3.0 * 8 * A.n * k);
-  if (k == 1 && A.nslices > 0 && A.s64.size()) {
+  if (k == 1 && A.nslices > 0 && A.s64.size() && A.sell_g == 1 && A.srow.size() == 0) {
     launch_sell_dot(h);
   } else if (k == 1) {
     switch (lanes_for(A)) {

@@ -341,14 +341,16 @@ __device__ __forceinline__ int64_t sell_row(const int32_t* __restrict__ srow, in
   return srow ? (int64_t)srow[p] : p;
 }
 
-__global__ void sell_width_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+// slices of 64 / g rows, g lanes per row: w[sl] = 64 * (steps of the longest row)
+__global__ void sell_width_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                   const int32_t* __restrict__ srow, int32_t* __restrict__ w) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
+  const int rows = 64 / g;
   int mx = 0;
-  for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n; ++p) {
+  for (int64_t p = sl * rows; p < sl * rows + rows && p < n; ++p) {
     const int64_t r = sell_row(srow, p);
-    mx = max(mx, rp[r + 1] - rp[r]);
+    mx = max(mx, (rp[r + 1] - rp[r] + g - 1) / g);
   }
   w[sl] = mx * 64;
 }
@@ -366,15 +368,16 @@ __global__ void sell_sort_keys_kernel(int64_t n, const int32_t* __restrict__ rp,
 // Window bases of each slice: the distinct aligned blocks (col >> kSellWinBits)
 // its columns touch, in order of first appearance (rows in order, entries in
 // row order).  More than kSellWindows blocks, or no entry at all: not compact.
-__global__ void sell_window_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+__global__ void sell_window_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                    const int32_t* __restrict__ ci, const int32_t* __restrict__ srow,
                                    int32_t* __restrict__ swin, int32_t* __restrict__ compact) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
+  const int rows = 64 / g;
   int blk[kSellWindows];
   int nb = 0;
   bool ok = true;
-  for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n && ok; ++p) {
+  for (int64_t p = sl * rows; p < sl * rows + rows && p < n && ok; ++p) {
     const int64_t r = sell_row(srow, p);
     for (int q = rp[r]; q < rp[r + 1]; ++q) {
       const int b = ci[q] >> kSellWinBits;
@@ -395,22 +398,24 @@ __global__ void sell_window_kernel(int64_t n, int64_t nslices, const int32_t* __
 }
 
 // stored nonzeros of each compact slice (0 for the others)
-__global__ void sell_compact_nnz_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+__global__ void sell_compact_nnz_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                         const int32_t* __restrict__ srow, const int32_t* __restrict__ compact,
                                         int64_t* __restrict__ out) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslices) return;
+  const int rows = 64 / g;
   int64_t cnt = 0;
   if (compact[sl])
-    for (int64_t p = sl * 64; p < sl * 64 + 64 && p < n; ++p) {
+    for (int64_t p = sl * rows; p < sl * rows + rows && p < n; ++p) {
       const int64_t r = sell_row(srow, p);
       cnt += rp[r + 1] - rp[r];
     }
   out[sl] = cnt;
 }
 
-// one thread per (slice, lane): row = slice * 64 + lane
-__global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+// one thread per (slice, lane): position = slice * (64 / g) + lane / g, and
+// the lane holds entries lane % g, lane % g + g, ... of that row
+__global__ void sell_fill_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                  const int32_t* __restrict__ ci, const double* __restrict__ v,
                                  const int32_t* __restrict__ srow, const int32_t* __restrict__ sp,
                                  const int32_t* __restrict__ swin, int32_t* __restrict__ sc,
@@ -420,7 +425,9 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
   if (t >= nslices * 64) return;
   int64_t sl = t >> 6;
   int lane = (int)(t & 63);
-  int64_t row = t < n ? sell_row(srow, t) : t;
+  const int sub = lane % g;
+  const int64_t pos = sl * (64 / g) + lane / g;
+  int64_t row = pos < n ? sell_row(srow, pos) : pos;
   int off = sp[sl];
   int w = (sp[sl + 1] - off) >> 6;
   int len = 0, base = 0;
@@ -436,7 +443,8 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
   int pad_col = len > 0 ? ci[base + len - 1] : (compact ? win[0] : 0);
   for (int k = 0; k < w; ++k) {
     int64_t e = off + (int64_t)k * 64 + lane;
-    const int col = k < len ? ci[base + k] : pad_col;
+    const int j = k * g + sub;
+    const int col = j < len ? ci[base + j] : pad_col;
     sc[e] = col;
     if (compact) {
       int slot = 0;
@@ -449,7 +457,7 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, const int32_t* __re
     } else {
       sc16[e] = 0;
     }
-    const double val = k < len ? v[base + k] : 0.0;
+    const double val = j < len ? v[base + j] : 0.0;
     if (s64) s64[e] = val;
     if (s32) s32[e] = (float)val;
     if (sh) sh[e] = half_bits(val);
@@ -553,9 +561,14 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
   HIP_LAUNCH_CHECK();
 }
 
-void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool with_f16, bool with_f64) {
-  const int64_t ns = (A.n + 63) / 64;
+void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool with_f16, bool with_f64,
+                int lanes) {
+  if (lanes < 1 || lanes > 32 || (lanes & (lanes - 1)) != 0)
+    fail(FEM_ERR_ARG, "sliced ELL: 1, 2, 4, 8, 16 or 32 lanes per row");
+  const int64_t rows = 64 / lanes;
+  const int64_t ns = (A.n + rows - 1) / rows;
   A.nslices = ns;
+  A.sell_g = lanes;
   A.sp.resize(ns + 1);
   A.srow.release();
   if (sort_window > 0 && A.n > 0) {
@@ -572,7 +585,7 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
   const int32_t* srow = A.srow.size() ? A.srow.get() : nullptr;
   DevBuf<int32_t> w(ns > 0 ? ns : 1);
   if (ns > 0) {
-    sell_width_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), srow, w.get());
+    sell_width_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), srow, w.get());
     HIP_LAUNCH_CHECK();
   }
   int64_t tot = dev_exclusive_scan_i32(w.get(), A.sp.get(), ns, s);
@@ -587,16 +600,17 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
   A.nnz_compact = 0;
   if (ns > 0) {
     DevBuf<int32_t> cflag(ns), cscan(ns + 1);
-    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), srow, A.swin.get(),
+    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow, A.swin.get(),
                                                          cflag.get());
     HIP_LAUNCH_CHECK();
     A.ncompact = dev_exclusive_scan_i32(cflag.get(), cscan.get(), ns, s);
     DevBuf<int64_t> snnz(ns), sscan(ns + 1);
-    sell_compact_nnz_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), srow, cflag.get(), snnz.get());
+    sell_compact_nnz_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), srow, cflag.get(),
+                                                              snnz.get());
     HIP_LAUNCH_CHECK();
     A.nnz_compact = dev_exclusive_scan_i64(snnz.get(), sscan.get(), ns, s);
-    sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), srow,
-                                                            A.sp.get(), A.swin.get(), A.sc.get(), A.sc16.get(),
+    sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), A.v64.get(),
+                                                            srow, A.sp.get(), A.swin.get(), A.sc.get(), A.sc16.get(),
                                                             with_f64 ? A.s64.get() : nullptr,
                                                             with_f32 ? A.s32.get() : nullptr,
                                                             with_f16 ? A.sh.get() : nullptr);

@@ -3,11 +3,13 @@ properties that do not need a direct solve: the true FP64 residual evaluated
 on the host from the exported matrix, agreement of the mixed and FP64 paths,
 linearity, the closed form 1^T K 1 = mass_coeff * vol(domain), symmetry, and
 the sliced-ELL operator (both its 16-bit-column and int32-column slices)
-against a host CSR product."""
+against a host CSR product, and every level matrix through the V-cycle's
+kernels against host products."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from conftest import report
 
@@ -93,6 +95,47 @@ def test_c3_operator_both_slice_kinds(fem, gpu, meshgen):
         ref = A @ X[:, c]
         err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
         assert err <= 1e-13, err
+    s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c3_level_kernels_match_host(fem, gpu, meshgen, precision):
+    """Every level's operator, prolongator and restriction applied through the
+    V-cycle's one-column kernels (sliced ELL with 1 to 8 lanes per row, or
+    CSR) against the host product of the exported FP64 matrices.  Bound per
+    row: (row length + 3) unit roundoffs of sum |m| |x| (values and x rounded
+    to the working precision, then a sum of row length products)."""
+    report(f"[gpu] 97^3 level kernels vs host products, precision={precision}")
+    xyz, tets, kappa, mass = _config(meshgen, "c3_cube97_mixed")
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    s.set_params(precision=precision, **PARAMS)
+    s.setup()
+    u = 2.0 ** -53 if precision == 0 else 2.0 ** -24
+    rng = np.random.default_rng(5)
+    nl = s.num_levels()
+    long_rows = 0
+    for l in range(nl):
+        rp, ci, v, _ = s.level_matrix(l)
+        n = rp.size - 1
+        mats = [(0, sp.csr_matrix((v, ci, rp), shape=(n, n)))]
+        if l < nl - 1:
+            nc = s.level_matrix(l + 1)[0].size - 1
+            prp, pci, pv = s.level_prolongator(l)
+            P = sp.csr_matrix((pv, pci, prp), shape=(n, nc))
+            mats += [(1, P), (2, P.T.tocsr())]
+        for which, M in mats:
+            x = rng.normal(size=M.shape[1])
+            y = s.apply_level(l, which, x)
+            ref = M @ x
+            bound = (np.diff(M.indptr) + 3) * u * (abs(M) @ abs(x)) + 1e-300
+            worst = float(np.max(np.abs(y - ref) / bound))
+            avg = M.nnz / max(M.shape[0], 1)
+            long_rows += M.shape[0] >= 32768 and avg > 24
+            report(f"      level {l} {'APR'[which]}: {M.shape[0]} x {M.shape[1]}, {avg:.1f} per row, "
+                   f"error / bound {worst:.2e}")
+            assert worst <= 1.0, (l, which, worst)
+    # the multi-lane sliced kernels ran (long rows on a large level)
+    assert long_rows >= 1
     s.close()
 
 
