@@ -346,11 +346,9 @@ __global__ void __launch_bounds__(kNT) pcg_spmv_dot_kernel(int64_t n, const int3
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int lane = threadIdx.x & (G - 1);
-  // block (XCD-remapped) owns a contiguous range of rows, kNT / G per pass
-  const int64_t per = kNT / G;
-  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
-  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
-  for (int64_t row = r0 + threadIdx.x / G; row < r1; row += per) {
+  // kNT / G rows per pass, one front per XCD
+  const XcdRows R = xcd_rows(n, kNT / G);
+  for (int64_t row = R.first + threadIdx.x / G; row < R.hi; row += R.step) {
     const int s = rp[row], e = rp[row + 1];
     double acc = 0.0;
     for (int j = s + lane; j < e; j += G) acc += ntload(val + j) * p[ntload(ci + j)];
@@ -396,12 +394,10 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
-  // block (XCD-remapped) owns a contiguous range of rows, kNT / KP per pass
-  const int64_t per = kNT / KP;
-  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
-  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
+  // kNT / KP rows per pass, one front per XCD
+  const XcdRows R = xcd_rows(n, kNT / KP);
   if (c < k) {
-    for (int64_t row = r0 + threadIdx.x / KP; row < r1; row += per) {
+    for (int64_t row = R.first + threadIdx.x / KP; row < R.hi; row += R.step) {
       const int s = rp[row], e = rp[row + 1];
       double acc = 0.0;
       int j = s;
@@ -422,11 +418,12 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_kernel(int64_t n, int k, con
 }
 
 // x += alpha p; r -= alpha q; partials of r.r
+// (and r32 = fl32(r), the FP32 V-cycle's input, when r32 is given)
 template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, double* __restrict__ x,
                                                        double* __restrict__ r, const double* __restrict__ p,
                                                        const double* __restrict__ q, double* __restrict__ partials,
-                                                       const double* __restrict__ scal) {
+                                                       const double* __restrict__ scal, float* __restrict__ r32) {
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
@@ -438,6 +435,7 @@ __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, doubl
       x[t] += a * p[t];
       double rv = r[t] - a * q[t];
       r[t] = rv;
+      if (r32) r32[t] = (float)rv;
       part += rv * rv;
     }
   }
@@ -455,12 +453,10 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_vec_kernel(int64_t n, int k,
   typedef double vec __attribute__((ext_vector_type(2)));
   __shared__ double s_buf[2 * kNT];
   const int c0 = (int)(threadIdx.x & (L - 1)) * 2;
-  const int64_t per = kNT / L;
-  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + per - 1) / per * per;
-  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk, r1 = min(n, r0 + chunk);
+  const XcdRows R = xcd_rows(n, kNT / L);
   vec part = (vec)(0.0);
   if (c0 < k) {
-    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += per) {
+    for (int64_t row = R.first + threadIdx.x / L; row < R.hi; row += R.step) {
       const int s = rp[row], e = rp[row + 1];
       vec acc = (vec)(0.0);
       int j = s;
@@ -490,16 +486,16 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_vec_kernel(int64_t n, int k,
 }
 
 // partials of a.b per column
-template <int KP>
+template <int KP, typename B>
 __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const double* __restrict__ a,
-                                                    const double* __restrict__ b, double* __restrict__ partials) {
+                                                    const B* __restrict__ b, double* __restrict__ partials) {
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
   const int64_t rows_per_grid = (int64_t)gridDim.x * (kNT / KP);
   if (c < k)
     for (int64_t row = ((int64_t)blockIdx.x * kNT + threadIdx.x) / KP; row < n; row += rows_per_grid)
-      part += a[row * k + c] * b[row * k + c];
+      part += a[row * k + c] * (double)b[row * k + c];
   block_col_partials<KP>(part, k, s_buf, partials);
 }
 
@@ -556,12 +552,15 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
 }
 
 // p = z + beta p
-__global__ void pcg_p_kernel(int64_t m, int k, const double* __restrict__ z, double* __restrict__ p,
+// p = z + beta p (z as the preconditioner left it: FP64, or the FP32 V-cycle
+// result, converted exactly)
+template <typename Z>
+__global__ void pcg_p_kernel(int64_t m, int k, const Z* __restrict__ z, double* __restrict__ p,
                              const double* __restrict__ scal) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
   int c = (int)(t % k);
-  p[t] = z[t] + scal[S_BETA * k + c] * p[t];
+  p[t] = (double)z[t] + scal[S_BETA * k + c] * p[t];
 }
 
 __global__ void copy_kernel(int64_t m, const double* __restrict__ a, double* __restrict__ b) {
@@ -853,41 +852,72 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
   return g;
 }
 
-// z = M r at level 0 (FP64 in and out); k columns.  With `partials` (one
-// column, mixed precision, level 0 smoothed before and after the coarse
-// correction, sliced ELL) the conversions are fused into the first and last
-// level-0 smoothing steps and the last one also writes per-block partials of
-// r.z there: the return value is their count (0 when not fused).
-unsigned precondition(Handle& h, const double* r, double* z, int k, double* partials = nullptr) {
+// Where one application of the preconditioner left z = M r.
+struct PrecOut {
+  const double* z64 = nullptr;  // FP64 z: the caller's buffer or a level-0 work vector
+  const float* z32 = nullptr;   // or the FP32 V-cycle result (exact in FP64)
+  unsigned nb = 0;              // r.z partials written by the fused level-0 ends
+};
+
+// One column, mixed precision, level 0 smoothed before and after the coarse
+// correction, sliced ELL: the FP64 <-> FP32 conversions are fused into the
+// first and last level-0 smoothing steps, and the last one also writes
+// per-block partials of r.z.
+bool fused_level0(const Handle& h, int k, bool partials) {
+  const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
+  return partials && h.H.precision >= 1 && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
+         h.A.nslices > 0 && h.A.s32.size() > 0 && h.A.sell_g == 1;
+}
+
+// z = M r at level 0 (FP64 r); k columns.  z is written only on the fused
+// path; otherwise it is left where the V-cycle put it (no copy).  r32_ready:
+// the level-0 FP32 right-hand side already holds fl32(r).
+PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* partials, bool r32_ready) {
   hipStream_t s = h.stream;
   Level& L0 = *h.H.levels[0];
   const int64_t m = h.A.n * k;
+  PrecOut o;
   if (h.H.precision == 0) {
-    double* res = vcycle<double>(h, 0, r, k);
-    copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
+    o.z64 = vcycle<double>(h, 0, r, k);
+    return o;
+  }
+  if (fused_level0(h, k, partials != nullptr)) {
+    if (h.H.precision == 2 && h.H.ah.nslices > 0) {
+      o.nb = vcycle0_half(h, r, z, partials);
+    } else {
+      Level0Io io;
+      io.r64 = r;
+      io.b32 = L0.b32.get();
+      io.z64 = z;
+      io.partials = partials;
+      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io);
+      if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
+      o.nb = io.nb;
+    }
+    o.z64 = z;
+    return o;
+  }
+  if (!r32_ready) {
+    convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
     HIP_LAUNCH_CHECK();
-    return 0;
   }
-  const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
-  const bool fuse = partials && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
-                    h.A.nslices > 0 && h.A.s32.size() > 0 && h.A.sell_g == 1;
-  if (fuse && h.H.precision == 2 && h.H.ah.nslices > 0) return vcycle0_half(h, r, z, partials);
-  if (fuse) {
-    Level0Io io;
-    io.r64 = r;
-    io.b32 = L0.b32.get();
-    io.z64 = z;
-    io.partials = partials;
-    float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io);
-    if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
-    return io.nb;
+  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k);
+  return o;
+}
+
+// z = M r into z (FP64); returns the count of r.z partials written (fused path)
+unsigned precondition(Handle& h, const double* r, double* z, int k, double* partials = nullptr) {
+  hipStream_t s = h.stream;
+  const int64_t m = h.A.n * k;
+  const PrecOut o = precondition_pcg(h, r, z, k, partials, false);
+  if (o.z32) {
+    convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, o.z32, z);
+    HIP_LAUNCH_CHECK();
+  } else if (o.z64 != z) {
+    copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, o.z64, z);
+    HIP_LAUNCH_CHECK();
   }
-  convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
-  HIP_LAUNCH_CHECK();
-  float* res = vcycle<float>(h, 0, L0.b32.get(), k);
-  convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, res, z);
-  HIP_LAUNCH_CHECK();
-  return 0;
+  return o.nb;
 }
 
 }  // namespace
@@ -1064,47 +1094,49 @@ void finish(Handle& h, int nb, int k, int mode, int slot, double tol2 = 0.0) {
   HIP_LAUNCH_CHECK();
 }
 
-template <int KP>
-void launch_dot(Handle& h, int k, const double* a, const double* b, int mode, int slot) {
+template <int KP, typename B>
+void launch_dot(Handle& h, int k, const double* a, const B* b, int mode, int slot) {
   const int64_t n = h.A.n;
   unsigned g = vec_grid(n, KP);
-  pcg_dot_kernel<KP><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get());
+  pcg_dot_kernel<KP, B><<<g, kNT, 0, h.stream>>>(n, k, a, b, h.partials.get());
   HIP_LAUNCH_CHECK();
   finish(h, (int)g, k, mode, slot);
 }
 
 // mode F_STORE: scal[slot] = a.b; F_BETA: beta = a.b / rz, rz = a.b
-void dot(Handle& h, int k, const double* a, const double* b, int mode, int slot) {
+template <typename B>
+void dot(Handle& h, int k, const double* a, const B* b, int mode, int slot) {
   switch (pow2_at_least(k)) {
-    case 1: launch_dot<1>(h, k, a, b, mode, slot); break;
-    case 2: launch_dot<2>(h, k, a, b, mode, slot); break;
-    case 4: launch_dot<4>(h, k, a, b, mode, slot); break;
-    case 8: launch_dot<8>(h, k, a, b, mode, slot); break;
-    case 16: launch_dot<16>(h, k, a, b, mode, slot); break;
-    case 32: launch_dot<32>(h, k, a, b, mode, slot); break;
-    default: launch_dot<64>(h, k, a, b, mode, slot); break;
+    case 1: launch_dot<1, B>(h, k, a, b, mode, slot); break;
+    case 2: launch_dot<2, B>(h, k, a, b, mode, slot); break;
+    case 4: launch_dot<4, B>(h, k, a, b, mode, slot); break;
+    case 8: launch_dot<8, B>(h, k, a, b, mode, slot); break;
+    case 16: launch_dot<16, B>(h, k, a, b, mode, slot); break;
+    case 32: launch_dot<32, B>(h, k, a, b, mode, slot); break;
+    default: launch_dot<64, B>(h, k, a, b, mode, slot); break;
   }
 }
 
 template <int KP>
-void launch_update(Handle& h, int k, double tol2) {
+void launch_update(Handle& h, int k, double tol2, float* r32) {
   const int64_t n = h.A.n;
   unsigned g = vec_grid(n, KP);
   pcg_update_kernel<KP><<<g, kNT, 0, h.stream>>>(n, k, h.kx.get(), h.kr.get(), h.kp.get(), h.kq.get(),
-                                                 h.partials.get(), h.scal.get());
+                                                 h.partials.get(), h.scal.get(), r32);
   HIP_LAUNCH_CHECK();
   finish(h, (int)g, k, F_RR, 0, tol2);
 }
 
-void update(Handle& h, int k, double tol2) {
+// x += alpha p, r -= alpha q, |r|^2, and r32 = fl32(r) when r32 is given
+void update(Handle& h, int k, double tol2, float* r32) {
   switch (pow2_at_least(k)) {
-    case 1: launch_update<1>(h, k, tol2); break;
-    case 2: launch_update<2>(h, k, tol2); break;
-    case 4: launch_update<4>(h, k, tol2); break;
-    case 8: launch_update<8>(h, k, tol2); break;
-    case 16: launch_update<16>(h, k, tol2); break;
-    case 32: launch_update<32>(h, k, tol2); break;
-    default: launch_update<64>(h, k, tol2); break;
+    case 1: launch_update<1>(h, k, tol2, r32); break;
+    case 2: launch_update<2>(h, k, tol2, r32); break;
+    case 4: launch_update<4>(h, k, tol2, r32); break;
+    case 8: launch_update<8>(h, k, tol2, r32); break;
+    case 16: launch_update<16>(h, k, tol2, r32); break;
+    case 32: launch_update<32>(h, k, tol2, r32); break;
+    default: launch_update<64>(h, k, tol2, r32); break;
   }
 }
 
@@ -1266,24 +1298,34 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       }
     } ev_guard{evs};
     int restarts = 0;
+    // the unfused mixed-precision V-cycle takes fl32(r), written by the update
+    float* r32 = (h.H.precision >= 1 && !fused_level0(h, k, true)) ? h.H.levels[0]->b32.get() : nullptr;
     while (any) {
-      // z = M r and rz = r.z (fused into the last level-0 smoothing step when possible)
-      auto precondition_dot = [&](int mode, int slot) {
-        const unsigned nb = precondition(h, h.kr.get(), h.kz.get(), k, h.partials.get());
-        if (nb) finish(h, (int)nb, k, mode, slot);
-        else dot(h, k, h.kr.get(), h.kz.get(), mode, slot);
+      // z = M r and rz = r.z (fused into the last level-0 smoothing step when
+      // possible); z stays where the preconditioner left it
+      auto precondition_dot = [&](int mode, int slot, bool r32_ready) {
+        const PrecOut o = precondition_pcg(h, h.kr.get(), h.kz.get(), k, h.partials.get(), r32_ready && r32);
+        if (o.nb) finish(h, (int)o.nb, k, mode, slot);
+        else if (o.z32) dot(h, k, h.kr.get(), o.z32, mode, slot);
+        else dot(h, k, h.kr.get(), o.z64, mode, slot);
+        return o;
       };
-      precondition_dot(F_STORE, S_RZ);
-      copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, h.kz.get(), h.kp.get());
+      const PrecOut o0 = precondition_dot(F_STORE, S_RZ, false);
+      if (o0.z32) convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, o0.z32, h.kp.get());
+      else copy_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, o0.z64, h.kp.get());
       HIP_LAUNCH_CHECK();
       // the first iteration starts from p = z; every later one is the fixed
-      // sequence below (captured once, then replayed)
+      // sequence below (captured once, then replayed), which follows an update
+      // (so fl32(r) is in place)
       auto iteration = [&]() {
-        precondition_dot(F_BETA, 0);  // z, beta, rz
-        pcg_p_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, h.kz.get(), h.kp.get(), h.scal.get());
+        const PrecOut o = precondition_dot(F_BETA, 0, true);  // z, beta, rz
+        if (o.z32)
+          pcg_p_kernel<float><<<grid_for(m, 256), 256, 0, s>>>(m, k, o.z32, h.kp.get(), h.scal.get());
+        else
+          pcg_p_kernel<double><<<grid_for(m, 256), 256, 0, s>>>(m, k, o.z64, h.kp.get(), h.scal.get());
         HIP_LAUNCH_CHECK();
-        spmv_dot(h, k);      // q = A p, alpha
-        update(h, k, tol2);  // x, r, rr, act
+        spmv_dot(h, k);           // q = A p, alpha
+        update(h, k, tol2, r32);  // x, r, rr, act
       };
       // Iteration n ends with the n-th |r|^2 finish of this solve; the host
       // reads its record while iteration n + 1 may already run: the next
@@ -1295,7 +1337,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       auto issue = [&](bool first) {
         if (first) {
           spmv_dot(h, k);
-          update(h, k, tol2);
+          update(h, k, tol2, r32);
         } else {
           run_iteration(h, k, tol2, iteration);
         }
