@@ -310,6 +310,148 @@ __global__ void __launch_bounds__(kNT) csr_multi_vec_kernel(int64_t n, int k, co
   }
 }
 
+// k right-hand sides, long rows: as csr_multi_vec_kernel, and ES lane groups
+// split the row's entries (group e takes entries e, e + ES, ...); a butterfly
+// over the groups adds their sums (fixed order).  L * ES lanes per row.
+template <typename T, int L, int ES, int MODE>
+__global__ void __launch_bounds__(kNT) csr_multi_split_kernel(int64_t n, int k, const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            const T* __restrict__ val, const T* __restrict__ x,
+                                                            T* __restrict__ y, const T* __restrict__ b,
+                                                            const T* __restrict__ dinv, T* __restrict__ d,
+                                                            T* __restrict__ xn, T c1, T c2) {
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int LR = L * ES;
+  static_assert(LR <= 64, "a row's lanes stay inside one wavefront");
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  const int64_t row = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kNT + threadIdx.x) / LR;
+  if (row >= n) return;  // all LR lanes of a row leave together
+  const int sub = threadIdx.x & (LR - 1);
+  const int eg = sub / L;
+  const int c0 = (sub % L) * VW;
+  const bool on = c0 < k;
+  vec acc = (vec)(T(0));
+  if (on) {
+    const int s = rp[row], e = rp[row + 1];
+    int q = s + eg;
+    for (; q + 3 * ES < e; q += 4 * ES) {
+      const int j0 = ci[q], j1 = ci[q + ES], j2 = ci[q + 2 * ES], j3 = ci[q + 3 * ES];
+      const T v0 = val[q], v1 = val[q + ES], v2 = val[q + 2 * ES], v3 = val[q + 3 * ES];
+      const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+      const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+      acc += v0 * g0;
+      acc += v1 * g1;
+      acc += v2 * g2;
+      acc += v3 * g3;
+    }
+    for (; q < e; q += ES) acc += val[q] * *(const vec*)(x + (int64_t)ci[q] * k + c0);
+  }
+#pragma unroll
+  for (int off = LR / 2; off >= L; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < VW; ++j) acc[j] += shfl_xor_t(acc[j], off, LR);
+  }
+  if (!on || eg != 0) return;
+  const int64_t idx = row * k + c0;
+  if (MODE == kAssign) {
+    *(vec*)(y + idx) = acc;
+  } else if (MODE == kResid) {
+    *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
+  } else if (MODE == kAdd) {
+    *(vec*)(y + idx) += acc;
+  } else if (MODE == kJacobi) {
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + (c1 * dinv[row]) * (*(const vec*)(b + idx) - acc);
+  } else {  // kCheb
+    const vec r = dinv[row] * (*(const vec*)(b + idx) - acc);
+    const vec dn = (c1 != T(0) ? c1 * *(const vec*)(d + idx) : (vec)(T(0))) + c2 * r;
+    *(vec*)(d + idx) = dn;
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + dn;
+  }
+}
+
+// k right-hand sides on the sliced-ELL copy with one lane per row in its
+// layout (64-row slices): L lanes per row, each owning VW = 16 / sizeof(T)
+// consecutive columns gathered as one 16-byte load; a slice takes L
+// wavefronts, each over 64 / L of its rows, so a step reads the 2-byte codes
+// and values of those rows as one contiguous run.  Entries in row order,
+// padding (value 0) last: the per-column sums are those of
+// csr_multi_vec_kernel.
+template <typename T, int L, int MODE, int VW = 16 / (int)sizeof(T)>
+__global__ void __launch_bounds__(kNT) sell_multi_kernel(int64_t n, int k, int64_t nslices, SellView S,
+                                                       const T* __restrict__ sv, const T* __restrict__ x,
+                                                       T* __restrict__ y, const T* __restrict__ b,
+                                                       const T* __restrict__ dinv, T* __restrict__ d,
+                                                       T* __restrict__ xn, T c1, T c2) {
+  constexpr int RW = 64 / L;  // rows per wavefront
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  const int gw = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
+  const int slice = gw / L;
+  if (slice >= nslices) return;
+  const int lane = threadIdx.x & 63;
+  const int rl = (gw % L) * RW + lane / L;  // row position inside the slice
+  const int c0 = (lane % L) * VW;
+  const int off = __builtin_amdgcn_readfirstlane(S.sp[slice]);
+  const int w = __builtin_amdgcn_readfirstlane((S.sp[slice + 1] - off) >> 6);
+  const int32_t* wp = S.swin + slice * kSellWindows;
+  const SellWin W{__builtin_amdgcn_readfirstlane(wp[0]), __builtin_amdgcn_readfirstlane(wp[1]),
+                  __builtin_amdgcn_readfirstlane(wp[2]), __builtin_amdgcn_readfirstlane(wp[3]),
+                  __builtin_amdgcn_readfirstlane(wp[4]), __builtin_amdgcn_readfirstlane(wp[5]),
+                  __builtin_amdgcn_readfirstlane(wp[6]), __builtin_amdgcn_readfirstlane(wp[7])};
+  const bool on = c0 < k;
+  vec acc = (vec)(T(0));
+  const T* vp = sv + off + rl;
+  if (on) {
+    if (W.b0 >= 0) {
+      const uint16_t* cp = S.sc16 + off + rl;
+      int q = 0;
+      for (; q + 4 <= w; q += 4) {
+        const int j0 = sell_col(cp[64 * q], W), j1 = sell_col(cp[64 * (q + 1)], W);
+        const int j2 = sell_col(cp[64 * (q + 2)], W), j3 = sell_col(cp[64 * (q + 3)], W);
+        const T v0 = vp[64 * q], v1 = vp[64 * (q + 1)], v2 = vp[64 * (q + 2)], v3 = vp[64 * (q + 3)];
+        const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+        const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+        acc += v0 * g0;
+        acc += v1 * g1;
+        acc += v2 * g2;
+        acc += v3 * g3;
+      }
+      for (; q < w; ++q) acc += vp[64 * q] * *(const vec*)(x + (int64_t)sell_col(cp[64 * q], W) * k + c0);
+    } else {
+      const int32_t* cp = S.sc + off + rl;
+      int q = 0;
+      for (; q + 4 <= w; q += 4) {
+        const int j0 = cp[64 * q], j1 = cp[64 * (q + 1)], j2 = cp[64 * (q + 2)], j3 = cp[64 * (q + 3)];
+        const T v0 = vp[64 * q], v1 = vp[64 * (q + 1)], v2 = vp[64 * (q + 2)], v3 = vp[64 * (q + 3)];
+        const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+        const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+        acc += v0 * g0;
+        acc += v1 * g1;
+        acc += v2 * g2;
+        acc += v3 * g3;
+      }
+      for (; q < w; ++q) acc += vp[64 * q] * *(const vec*)(x + (int64_t)cp[64 * q] * k + c0);
+    }
+  }
+  const int64_t pos = (int64_t)slice * 64 + rl;
+  if (!on || pos >= n) return;
+  const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
+  const int64_t idx = row * k + c0;
+  if (MODE == kAssign) {
+    *(vec*)(y + idx) = acc;
+  } else if (MODE == kResid) {
+    *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
+  } else if (MODE == kAdd) {
+    *(vec*)(y + idx) += acc;
+  } else if (MODE == kJacobi) {
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + (c1 * dinv[row]) * (*(const vec*)(b + idx) - acc);
+  } else {  // kCheb
+    const vec r = dinv[row] * (*(const vec*)(b + idx) - acc);
+    const vec dn = (c1 != T(0) ? c1 * *(const vec*)(d + idx) : (vec)(T(0))) + c2 * r;
+    *(vec*)(d + idx) = dn;
+    *(vec*)(xn + idx) = *(const vec*)(x + idx) + dn;
+  }
+}
+
 // ---- reductions ---------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {

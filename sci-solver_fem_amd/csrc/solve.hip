@@ -93,9 +93,35 @@ void launch_vec(const Csr& A, const V* val, const T* x, T* y, const T* b, const 
   csr_vec_kernel<V, T, G, MODE><<<grid, kNT, 0, s>>>(A.n, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn, c1, c2);
 }
 
+template <typename T, int MODE, int L, int ES>
+void launch_multi_split(const Csr& A, int k, const T* val, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn,
+                        T c1, T c2, hipStream_t s) {
+  unsigned grid = grid_for(A.n * L * ES, kNT);
+  csr_multi_split_kernel<T, L, ES, MODE><<<grid, kNT, 0, s>>>(A.n, k, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d,
+                                                              xn, c1, c2);
+}
+
+// Entry groups per row of the multi-column CSR kernels: enough for at most
+// ~16 entries each (coarse operators, restrictions), within one wavefront.
+int entry_split(const Csr& A, int L) {
+  const double avg = A.n ? (double)A.nnz / (double)A.n : 0.0;
+  int es = 1;
+  while (es * L < 64 && avg > 16.0 * es) es *= 2;
+  return es;
+}
+
 template <typename T, int MODE, int L>
 void launch_multi_vec(const Csr& A, int k, const T* val, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn,
                       T c1, T c2, hipStream_t s) {
+  switch (L < 64 ? entry_split(A, L) : 1) {
+    case 1: break;
+    case 2: launch_multi_split<T, MODE, L, (L <= 32 ? 2 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+    case 4: launch_multi_split<T, MODE, L, (L <= 16 ? 4 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+    case 8: launch_multi_split<T, MODE, L, (L <= 8 ? 8 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+    case 16: launch_multi_split<T, MODE, L, (L <= 4 ? 16 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+    case 32: launch_multi_split<T, MODE, L, (L <= 2 ? 32 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+    default: launch_multi_split<T, MODE, L, (L <= 1 ? 64 : 1)>(A, k, val, x, y, b, dinv, d, xn, c1, c2, s); return;
+  }
   unsigned grid = grid_for(A.n * L, kNT);
   csr_multi_vec_kernel<T, L, MODE><<<grid, kNT, 0, s>>>(A.n, k, A.rp.get(), A.ci.get(), val, x, y, b, dinv, d, xn,
                                                         c1, c2);
@@ -122,6 +148,17 @@ void launch_sell(const Csr& A, const T* x, T* y, const T* b, const T* dinv, T* d
                                                                        y, b, dinv, d, xn, c1, c2);
 }
 
+// columns per lane of the multi-column sliced kernel: 4 (16 bytes FP32, 32 bytes FP64)
+constexpr int kSellMultiCols = 4;
+
+template <typename T, int MODE, int L>
+void launch_sell_multi(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn, T c1, T c2,
+                       hipStream_t s) {
+  unsigned grid = grid_for(A.nslices * L, kNT / 64);
+  sell_multi_kernel<T, L, MODE, kSellMultiCols><<<grid, kNT, 0, s>>>(A.n, k, A.nslices, sell_view(A), svals_of<T>(A),
+                                                                     x, y, b, dinv, d, xn, c1, c2);
+}
+
 // y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.  One RHS
 // uses the sliced-ELL copy when it exists, otherwise CSR with G lanes per row.
 template <typename T, int MODE>
@@ -145,6 +182,16 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
       case 16: launch_vec<T, T, MODE, 16>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       case 32: launch_vec<T, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       default: launch_vec<T, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
+    }
+  } else if (k % kSellMultiCols == 0 && A.nslices > 0 && A.sell_g == 1 && svals_of<T>(A)) {
+    // groups of 4 columns on the sliced-ELL copy
+    switch (pow2_at_least(k / kSellMultiCols)) {
+      case 1: launch_sell_multi<T, MODE, 1>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 2: launch_sell_multi<T, MODE, 2>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 4: launch_sell_multi<T, MODE, 4>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 8: launch_sell_multi<T, MODE, 8>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
+      case 16: launch_sell_multi<T, MODE, 16>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
+      default: launch_sell_multi<T, MODE, 32>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
   } else if (k % (16 / (int)sizeof(T)) == 0) {
     // 16-byte column groups (all multi-column vectors here come from hipMalloc)
@@ -481,6 +528,71 @@ __global__ void __launch_bounds__(kNT) pcg_spmm_dot_vec_kernel(int64_t n, int k,
     const int c = threadIdx.x;
     double sum = 0.0;
     for (int t = c / 2; t < kNT; t += L) sum += s_buf[2 * t + (c & 1)];
+    partials[(int64_t)blockIdx.x * k + c] = sum;
+  }
+}
+
+// q = A p on the sliced-ELL copy (one lane per row in its layout) for an even
+// k, with per-block, per-column partials of p.q.  As sell_multi_kernel, L
+// lanes per row own two columns each; a unit is one slice quarter (64 / L
+// rows), walked by the block's wavefronts one front per XCD.
+template <int L, int CW>
+__global__ void __launch_bounds__(kNT) pcg_sell_multi_dot_kernel(int64_t n, int k, int64_t nslices, SellView S,
+                                                                const double* __restrict__ sv,
+                                                                const double* __restrict__ p, double* __restrict__ q,
+                                                                double* __restrict__ partials) {
+  typedef double vec __attribute__((ext_vector_type(CW)));
+  constexpr int RW = 64 / L;
+  __shared__ double s_buf[CW * kNT];
+  const int lane = threadIdx.x & 63;
+  const int c0 = (lane % L) * CW;
+  const bool on = c0 < k;
+  const XcdRows R = xcd_rows(nslices * L, kNT / 64);
+  vec part = (vec)(0.0);
+  for (int64_t u = R.first + (threadIdx.x >> 6); u < R.hi; u += R.step) {
+    const int unit = __builtin_amdgcn_readfirstlane((int)u);
+    const int slice = unit / L;
+    const int rl = (unit % L) * RW + lane / L;
+    const int off = __builtin_amdgcn_readfirstlane(S.sp[slice]);
+    const int w = __builtin_amdgcn_readfirstlane((S.sp[slice + 1] - off) >> 6);
+    const int32_t* wp = S.swin + slice * kSellWindows;
+    const SellWin W{__builtin_amdgcn_readfirstlane(wp[0]), __builtin_amdgcn_readfirstlane(wp[1]),
+                    __builtin_amdgcn_readfirstlane(wp[2]), __builtin_amdgcn_readfirstlane(wp[3]),
+                    __builtin_amdgcn_readfirstlane(wp[4]), __builtin_amdgcn_readfirstlane(wp[5]),
+                    __builtin_amdgcn_readfirstlane(wp[6]), __builtin_amdgcn_readfirstlane(wp[7])};
+    const int64_t pos = (int64_t)slice * 64 + rl;
+    if (!on || pos >= n) continue;
+    const double* vp = sv + off + rl;
+    vec acc = (vec)(0.0);
+    if (W.b0 >= 0) {
+      const uint16_t* cp = S.sc16 + off + rl;
+      int j = 0;
+      for (; j + 4 <= w; j += 4) {
+        const int j0 = sell_col(cp[64 * j], W), j1 = sell_col(cp[64 * (j + 1)], W);
+        const int j2 = sell_col(cp[64 * (j + 2)], W), j3 = sell_col(cp[64 * (j + 3)], W);
+        const double v0 = vp[64 * j], v1 = vp[64 * (j + 1)], v2 = vp[64 * (j + 2)], v3 = vp[64 * (j + 3)];
+        const vec g0 = *(const vec*)(p + (int64_t)j0 * k + c0), g1 = *(const vec*)(p + (int64_t)j1 * k + c0);
+        const vec g2 = *(const vec*)(p + (int64_t)j2 * k + c0), g3 = *(const vec*)(p + (int64_t)j3 * k + c0);
+        acc += v0 * g0;
+        acc += v1 * g1;
+        acc += v2 * g2;
+        acc += v3 * g3;
+      }
+      for (; j < w; ++j) acc += vp[64 * j] * *(const vec*)(p + (int64_t)sell_col(cp[64 * j], W) * k + c0);
+    } else {
+      const int32_t* cp = S.sc + off + rl;
+      for (int j = 0; j < w; ++j) acc += vp[64 * j] * *(const vec*)(p + (int64_t)cp[64 * j] * k + c0);
+    }
+    *(vec*)(q + pos * k + c0) = acc;
+    part += acc * *(const vec*)(p + pos * k + c0);
+  }
+#pragma unroll
+  for (int j = 0; j < CW; ++j) s_buf[CW * threadIdx.x + j] = part[j];
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    const int c = threadIdx.x;
+    double sum = 0.0;
+    for (int t = c / CW; t < kNT; t += L) sum += s_buf[CW * t + (c % CW)];
     partials[(int64_t)blockIdx.x * k + c] = sum;
   }
 }
@@ -1162,12 +1274,24 @@ void launch_spmv_dot(Handle& h) {
   finish(h, (int)g, 1, F_ALPHA, 0);
 }
 
+#ifndef FEM_DOT_COLS
+#define FEM_DOT_COLS 4
+#endif
+constexpr int kDotCols = FEM_DOT_COLS;  // TUNE (temporary macro)
+
 template <int KP>
 void launch_spmm_dot(Handle& h, int k) {
   const Csr& A = h.A;
   int64_t g = (A.n * KP + kNT - 1) / kNT;
   if (g > 4096) g = 4096;
-  if (k % 2 == 0) {
+  if (k % kDotCols == 0 && A.nslices > 0 && A.sell_g == 1 && A.s64.size() && A.srow.size() == 0) {
+    constexpr int L = KP >= kDotCols ? KP / kDotCols : 1;
+    int64_t gs = (A.nslices * L + kNT / 64 - 1) / (kNT / 64);
+    if (gs > 4096) gs = 4096;
+    g = gs;
+    pcg_sell_multi_dot_kernel<L, kDotCols><<<(unsigned)g, kNT, 0, h.stream>>>(
+        A.n, k, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(), h.partials.get());
+  } else if (k % 2 == 0) {
     // KP >= k, so KP / 2 lanes per row cover the k / 2 column pairs
     pcg_spmm_dot_vec_kernel<(KP >= 2 ? KP / 2 : 1)><<<(unsigned)g, kNT, 0, h.stream>>>(
         A.n, k, A.rp.get(), A.ci.get(), A.v64.get(), h.kp.get(), h.kq.get(), h.partials.get());
