@@ -235,9 +235,11 @@ def test_queued_ahead_iteration_leaves_solution_unchanged(fem, gpu, oracle, mesh
 @pytest.mark.parametrize("k", [3, 8, 16])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batched_widths(fem, gpu, oracle, meshgen, k, precision):
-    """k columns take the one-column-per-lane kernels (k = 3) or the 16-byte
-    column-group kernels (k a multiple of 4 floats / 2 doubles); every column
-    must match its single-column solve."""
+    """k columns take the one-column-per-lane kernels (k = 3), the 4-column
+    group kernels on the sliced-ELL copies (fine operator, prolongators; FP32
+    and FP64, operator + dot included) or the CSR column-group kernels with
+    split entries (long coarse rows, restrictions); every column must match
+    its single-column solve."""
     report(f"[gpu] batched k={k} precision={precision}")
     xyz, conn, kappa, A, _ = _problem(meshgen, oracle, 11, 1.5, 1e-2, seed=8)
     s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
