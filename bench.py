@@ -99,7 +99,7 @@ def cpu_baseline(fem, size: int):
             "converged": res["converged"], "rel_residual": res["rel_residual"]}
 
 
-DOMINANT_KERNEL = "sell_kernel<float, 4>"  # level-0 Chebyshev step (FP32 sliced ELL, fused epilogue)
+DOMINANT_KERNEL = "sell_kernel<float, 4, 1>"  # level-0 Chebyshev step (FP32 sliced ELL, fused epilogue)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_level0_chebyshev_step.json"
 
 

@@ -148,15 +148,18 @@ void launch_sell(const Csr& A, const T* x, T* y, const T* b, const T* dinv, T* d
                                                                        y, b, dinv, d, xn, c1, c2);
 }
 
-// columns per lane of the multi-column sliced kernel: 4 (16 bytes FP32, 32 bytes FP64)
-constexpr int kSellMultiCols = 4;
+// columns per lane of the multi-column sliced kernel: 4 (FP32 16-byte, FP64
+// 32-byte gathers; measured at 161^3 with 16 columns: FP64 with 2 per lane
+// 12 % slower, FP32 with 8 per lane 7 % slower)
+template <typename T>
+constexpr int sell_multi_cols() { return 4; }
 
 template <typename T, int MODE, int L>
 void launch_sell_multi(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn, T c1, T c2,
                        hipStream_t s) {
   unsigned grid = grid_for(A.nslices * L, kNT / 64);
-  sell_multi_kernel<T, L, MODE, kSellMultiCols><<<grid, kNT, 0, s>>>(A.n, k, A.nslices, sell_view(A), svals_of<T>(A),
-                                                                     x, y, b, dinv, d, xn, c1, c2);
+  sell_multi_kernel<T, L, MODE, sell_multi_cols<T>()><<<grid, kNT, 0, s>>>(A.n, k, A.nslices, sell_view(A),
+                                                                           svals_of<T>(A), x, y, b, dinv, d, xn, c1, c2);
 }
 
 // y/xn = f(A x) for k right-hand sides; MODE selects the epilogue.  One RHS
@@ -183,9 +186,9 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
       case 32: launch_vec<T, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       default: launch_vec<T, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
-  } else if (k % kSellMultiCols == 0 && A.nslices > 0 && A.sell_g == 1 && svals_of<T>(A)) {
-    // groups of 4 columns on the sliced-ELL copy
-    switch (pow2_at_least(k / kSellMultiCols)) {
+  } else if (k % sell_multi_cols<T>() == 0 && A.nslices > 0 && A.sell_g == 1 && svals_of<T>(A)) {
+    // column groups on the sliced-ELL copy
+    switch (pow2_at_least(k / sell_multi_cols<T>())) {
       case 1: launch_sell_multi<T, MODE, 1>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
       case 2: launch_sell_multi<T, MODE, 2>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
       case 4: launch_sell_multi<T, MODE, 4>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
@@ -1274,10 +1277,8 @@ void launch_spmv_dot(Handle& h) {
   finish(h, (int)g, 1, F_ALPHA, 0);
 }
 
-#ifndef FEM_DOT_COLS
-#define FEM_DOT_COLS 4
-#endif
-constexpr int kDotCols = FEM_DOT_COLS;  // TUNE (temporary macro)
+// FP64 columns per lane of the multi-column sliced operator + dot (as sell_multi_cols)
+constexpr int kDotCols = 4;
 
 template <int KP>
 void launch_spmm_dot(Handle& h, int k) {
