@@ -22,6 +22,9 @@ enum SpmvMode : int {
   kAdd = 2,      // y += A x
   kJacobi = 3,   // xn = x + c1 * dinv * (b - A x)
   kCheb = 4,     // r = dinv (b - A x); d = c1 d + c2 r; xn = x + d
+  kAssignScale = 5,  // y = A x; xn = c1 * dinv * y; d = xn when d != null (a
+                     // restriction that also takes the next level's first
+                     // smoothing step from zero)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -87,6 +90,11 @@ __device__ __forceinline__ void spmv_epilogue(int64_t idx, int64_t row, T acc, T
                                               T* __restrict__ xn, T c1, T c2) {
   if (MODE == kAssign) {
     y[idx] = acc;
+  } else if (MODE == kAssignScale) {
+    y[idx] = acc;
+    const T v = c1 * (T)dinv[row] * acc;
+    xn[idx] = v;
+    if (d) d[idx] = v;
   } else if (MODE == kResid) {
     y[idx] = b[idx] - acc;
   } else if (MODE == kAdd) {
@@ -296,6 +304,11 @@ __global__ void __launch_bounds__(kNT) csr_multi_vec_kernel(int64_t n, int k, co
   const int64_t idx = row * k + c0;
   if (MODE == kAssign) {
     *(vec*)(y + idx) = acc;
+  } else if (MODE == kAssignScale) {
+    *(vec*)(y + idx) = acc;
+    const vec v = (c1 * (T)dinv[row]) * acc;
+    *(vec*)(xn + idx) = v;
+    if (d) *(vec*)(d + idx) = v;
   } else if (MODE == kResid) {
     *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
   } else if (MODE == kAdd) {
@@ -355,6 +368,11 @@ __global__ void __launch_bounds__(kNT) csr_multi_split_kernel(int64_t n, int k, 
   const int64_t idx = row * k + c0;
   if (MODE == kAssign) {
     *(vec*)(y + idx) = acc;
+  } else if (MODE == kAssignScale) {
+    *(vec*)(y + idx) = acc;
+    const vec v = (c1 * (T)dinv[row]) * acc;
+    *(vec*)(xn + idx) = v;
+    if (d) *(vec*)(d + idx) = v;
   } else if (MODE == kResid) {
     *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
   } else if (MODE == kAdd) {
@@ -438,6 +456,11 @@ __global__ void __launch_bounds__(kNT) sell_multi_kernel(int64_t n, int k, int64
   const int64_t idx = row * k + c0;
   if (MODE == kAssign) {
     *(vec*)(y + idx) = acc;
+  } else if (MODE == kAssignScale) {
+    *(vec*)(y + idx) = acc;
+    const vec v = (c1 * (T)dinv[row]) * acc;
+    *(vec*)(xn + idx) = v;
+    if (d) *(vec*)(d + idx) = v;
   } else if (MODE == kResid) {
     *(vec*)(y + idx) = *(const vec*)(b + idx) - acc;
   } else if (MODE == kAdd) {

@@ -746,9 +746,11 @@ bool fused_out(const Level0Io* io, const Csr& A, int k) {
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
 // Buffers x and t ping-pong; returns the buffer that holds the result (with
 // io->z64 set, the last step writes there instead and nullptr is returned).
+// first_done: the first step from zero is already in x (and d), taken by the
+// restriction that produced b.
 template <typename T>
 T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, bool zero, int k,
-          Level0Io* io = nullptr) {
+          Level0Io* io = nullptr, bool first_done = false) {
   Level& L = *h.H.levels[l];
   hipStream_t s = h.stream;
   const T* dinv = dinv_of<T>(L);
@@ -794,7 +796,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   if (S.type == 0) {
     const T w = (T)S.omega;
     if (zero) {
-      first_step(cur, nullptr, S.omega);
+      if (!first_done) first_step(cur, nullptr, S.omega);
       step = 1;
     }
     for (; step < nu; ++step) {
@@ -814,7 +816,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   const double sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   if (zero) {
-    first_step(cur, d, 1.0 / theta);
+    if (!first_done) first_step(cur, d, 1.0 / theta);
     step = 1;
   }
   for (; step < nu; ++step) {
@@ -840,10 +842,30 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   return cur;
 }
 
+// b_{l+1} = R r.  When level l+1 is smoothed from zero (not the coarsest
+// level) the same pass takes its first smoothing step, x = c D^-1 b (and
+// d = x for Chebyshev), as smooth() would; returns whether it did.
+template <typename T>
+bool restrict_first(Handle& h, int l, const Csr& R, const T* r, int k) {
+  Hierarchy& H = h.H;
+  Level& N = *H.levels[l + 1];
+  T* bc = LV<T>::b(N);
+  if (l + 1 == H.coarse || h.prm.preInnerIters < 1) {
+    spmv_op<T, kAssign>(R, k, r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), h.stream);
+    return false;
+  }
+  const Smoother S = smoother_for(h, N);
+  const double c = S.type ? 1.0 / (0.5 * (S.lmax + S.lmin)) : S.omega;
+  spmv_op<T, kAssignScale>(R, k, r, bc, nullptr, dinv_of<T>(N), S.type ? LV<T>::d(N) : nullptr, LV<T>::x(N), (T)c,
+                           T(0), h.stream);
+  return true;
+}
+
 // V-cycle below level l for the right-hand side b (level l vectors, k columns);
 // returns the buffer holding the correction (nullptr when io->z64 received it).
+// first_done: restrict_first took the first pre-smoothing step.
 template <typename T>
-T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr) {
+T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr, bool first_done = false) {
   Hierarchy& H = h.H;
   Level& L = *H.levels[l];
   hipStream_t s = h.stream;
@@ -859,16 +881,15 @@ T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr) {
   T* d = LV<T>::d(L);
   const int nu_pre = h.prm.preInnerIters;
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
-  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k, io);
+  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k, io, first_done);
   T* oth = (cur == x) ? t : x;
   {
     ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", matrix_bytes<T>(A, k) + 3.0 * A.n * k * sizeof(T));
     spmv_op<T, kResid>(A, k, cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   }
   Level& N = *H.levels[l + 1];
-  T* bc = LV<T>::b(N);
-  spmv_op<T, kAssign>(L.R, k, r, bc, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
-  T* xc = vcycle<T>(h, l + 1, bc, k);
+  const bool next_first = restrict_first<T>(h, l, L.R, r, k);
+  T* xc = vcycle<T>(h, l + 1, LV<T>::b(N), k, nullptr, next_first);
   spmv_op<T, kAdd>(L.P, k, xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k, io);
   return cur;
@@ -942,9 +963,8 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
                                               nullptr, 0.0f, 0.0f);
   HIP_LAUNCH_CHECK();
   Level& N = *H.levels[1];
-  float* bc = N.b32.get();
-  spmv_op<float, kAssign>(H.rh, 1, rr, bc, nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
-  float* xc = vcycle<float>(h, 1, bc, 1);
+  const bool next_first = restrict_first<float>(h, 0, H.rh, rr, 1);
+  float* xc = vcycle<float>(h, 1, N.b32.get(), 1, nullptr, next_first);
   spmv_op<float, kAdd>(H.ph, 1, xc, cur, nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
   // post-smoothing; the last step writes z and the r.z partials
   ChebSeq cp(S);
