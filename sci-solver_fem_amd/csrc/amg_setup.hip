@@ -15,6 +15,8 @@
 // deterministic function of (graph, seed) and matches the oracle bit for bit.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -514,6 +516,113 @@ __global__ void strength_kernel(int64_t n, const int32_t* __restrict__ rp, const
   if (!kFill) cnt[i] = c;
 }
 
+// Pattern of A without the diagonal (the aggregation graph on the full pattern).
+template <bool kFill>
+__global__ void offdiag_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                               int32_t* __restrict__ cnt, const int32_t* __restrict__ gp, int32_t* __restrict__ gj) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  const int o = kFill ? gp[i] : 0;
+  for (int q = rp[i]; q < rp[i + 1]; ++q) {
+    const int j = ci[q];
+    if (j == (int)i) continue;
+    if (kFill) gj[o + c] = j;
+    ++c;
+  }
+  if (!kFill) cnt[i] = c;
+}
+
+void offdiag_graph(const Csr& A, Csr& G, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<int32_t> cnt(n > 0 ? n : 1);
+  if (n > 0) {
+    offdiag_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), cnt.get(), nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+  }
+  G.n = G.m = n;
+  G.rp.resize(n + 1);
+  G.nnz = dev_exclusive_scan_i32(cnt.get(), G.rp.get(), n, s);
+  G.ci.resize(G.nnz > 0 ? G.nnz : 1);
+  if (n > 0) {
+    offdiag_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), nullptr, G.rp.get(), G.ci.get());
+    HIP_LAUNCH_CHECK();
+  }
+}
+
+// Prolongator pattern for rows of at most kAggRowCap entries: row i holds the
+// sorted distinct aggregates agg(j) of the columns j of row i of A (what the
+// sparse product with the aggregate map gives), by insertion into a sorted
+// list in LDS.  Longer rows are flagged in *over.
+constexpr int kAggRowCap = 32;
+constexpr int kAggNT = 128;
+
+template <bool kFill>
+__global__ void __launch_bounds__(kAggNT) agg_pattern_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            const int32_t* __restrict__ agg, int32_t* __restrict__ cnt,
+                                                            const int32_t* __restrict__ pp, int32_t* __restrict__ pj,
+                                                            int* __restrict__ over) {
+  __shared__ int32_t buf[kAggNT * kAggRowCap];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t* lst = buf + threadIdx.x * kAggRowCap;
+  const int s0 = rp[i], e0 = rp[i + 1];
+  if (e0 - s0 > kAggRowCap) {
+    if (!kFill) {
+      cnt[i] = 0;
+      over[0] = 1;
+    }
+    return;
+  }
+  int c = 0;
+  for (int q = s0; q < e0; ++q) {
+    const int a = agg[ci[q]];
+    int pos = 0;
+    while (pos < c && lst[pos] < a) ++pos;
+    if (pos < c && lst[pos] == a) continue;
+    for (int t = c; t > pos; --t) lst[t] = lst[t - 1];
+    lst[pos] = a;
+    ++c;
+  }
+  if (kFill) {
+    const int o = pp[i];
+    for (int t = 0; t < c; ++t) pj[o + t] = lst[t];
+  } else {
+    cnt[i] = c;
+  }
+}
+
+// pattern of the prolongator (rows of A mapped through agg, sorted distinct);
+// false when a row of A is longer than kAggRowCap (the caller then uses the
+// general symbolic product)
+bool agg_pattern(const Csr& A, const int32_t* agg, int64_t nagg, Csr& P, hipStream_t s) {
+  const int64_t n = A.n;
+  DevBuf<int32_t> cnt(n > 0 ? n : 1);
+  DevBuf<int> over(1);
+  dev_fill_i32(over.get(), 0, 1, s);
+  if (n > 0) {
+    agg_pattern_kernel<false><<<grid_for(n, kAggNT), kAggNT, 0, s>>>(n, A.rp.get(), A.ci.get(), agg, cnt.get(), nullptr,
+                                                                    nullptr, over.get());
+    HIP_LAUNCH_CHECK();
+  }
+  int ho = 0;
+  HIP_CHECK(hipMemcpyAsync(&ho, over.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (ho) return false;
+  P.n = n;
+  P.m = nagg;
+  P.rp.resize(n + 1);
+  P.nnz = dev_exclusive_scan_i32(cnt.get(), P.rp.get(), n, s);
+  P.ci.resize(P.nnz > 0 ? P.nnz : 1);
+  if (n > 0) {
+    agg_pattern_kernel<true><<<grid_for(n, kAggNT), kAggNT, 0, s>>>(n, A.rp.get(), A.ci.get(), agg, nullptr,
+                                                                   P.rp.get(), P.ci.get(), nullptr);
+    HIP_LAUNCH_CHECK();
+  }
+  return true;
+}
+
 void strength_graph(const Csr& A, const double* dinv, double theta, Csr& G, hipStream_t s) {
   const int64_t n = A.n;
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
@@ -749,8 +858,18 @@ void build_half_level0(Handle& h) {
   sell_or_csr(H.rh, 0, sell_lanes(H.rh), true, s);
 }
 
+static void tmark(hipStream_t s, const char* what) {  // TUNE (temporary)
+  static auto last = std::chrono::steady_clock::now();
+  if (!getenv("FEM_SETUP_TIMES")) return;
+  (void)hipStreamSynchronize(s);
+  auto now = std::chrono::steady_clock::now();
+  fprintf(stderr, "[setup] %-28s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+  last = now;
+}
+
 void build_hierarchy(Handle& h) {
   auto t0 = std::chrono::steady_clock::now();
+  tmark(h.stream, "start");
   ++h.gen;  // invalidates a captured PCG iteration
   hipStream_t s = h.stream;
   Hierarchy& H = h.H;
@@ -765,7 +884,8 @@ void build_hierarchy(Handle& h) {
 
   // level-0 graph: pattern of A without the diagonal
   Csr G;
-  spgemm_symbolic(h.A, nullptr, nullptr, 0, nullptr, h.A.n, true, G, s);
+  if (h.prm.strengthThreshold <= 0.0) offdiag_graph(h.A, G, s);
+    tmark(s, "graph pattern");
 
   int64_t total_rows = 0, total_nnz = 0;
   std::unique_ptr<Level> cur(new Level());
@@ -787,20 +907,23 @@ void build_hierarchy(Handle& h) {
       if (hb) fail(FEM_ERR_NUMERIC, "level " + std::to_string(l) + " has " + std::to_string(hb) +
                                         " rows without a positive diagonal");
     }
+    tmark(s, "diag");
     const bool last = (n < h.prm.topSize) || (l + 1 >= h.prm.maxLevels) || n <= 1;
     int64_t nagg = 0;
     const double theta = h.prm.strengthThreshold;
     if (!last && theta > 0.0) strength_graph(A, L.dinv64.get(), theta, G, s);
+    tmark(s, "strength");
     if (!last) {
       L.agg.resize(n);
       unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
       aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
                       &nagg, s);
+      tmark(s, "aggregate");
       if (theta > 0.0 && 2 * nagg > n) {
         // The strength graph fell apart into small components (each left as
         // singletons after runt removal): coarsening would stall for many
         // levels.  Aggregate this level on the full pattern instead.
-        spgemm_symbolic(A, nullptr, nullptr, 0, nullptr, n, true, G, s);
+        offdiag_graph(A, G, s);
         aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
                         &nagg, s);
       }
@@ -833,22 +956,28 @@ void build_hierarchy(Handle& h) {
     }
     L.nc = nagg;
     L.rho = estimate_rho(A, L.dinv64.get(), s);
+    tmark(s, "rho");
     // prolongator pattern: columns agg(k) for k in row i of A
-    spgemm_symbolic(A, nullptr, nullptr, 0, L.agg.get(), nagg, false, L.P, s);
+    if (!agg_pattern(A, L.agg.get(), nagg, L.P, s))
+      spgemm_symbolic(A, nullptr, nullptr, 0, L.agg.get(), nagg, false, L.P, s);
     L.P.v64.resize(L.P.nnz);
     prolongator_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), L.dinv64.get(),
                                                         L.agg.get(), omega, L.P.rp.get(), L.P.ci.get(),
                                                         L.P.v64.get());
     HIP_LAUNCH_CHECK();
+    tmark(s, "P");
     csr_transpose(L.P, L.R, s);
+    tmark(s, "R");
     // A_c = R (A P)
     std::unique_ptr<Level> next(new Level());
     {
       Csr AP;
       spgemm_symbolic(A, L.P.rp.get(), L.P.ci.get(), L.P.n, nullptr, nagg, false, AP, s);
       spgemm_numeric(A, L.P, AP, s);
+    tmark(s, "AP");
       spgemm_symbolic(L.R, AP.rp.get(), AP.ci.get(), AP.n, nullptr, nagg, false, next->A, s);
       spgemm_numeric(L.R, AP, next->A, s);
+    tmark(s, "RAP");
     }
     // next graph: induced aggregate graph (aggregates sharing an edge); with a
     // strength threshold the next level builds its own strength graph instead
@@ -862,6 +991,7 @@ void build_hierarchy(Handle& h) {
     H.levels.push_back(std::move(cur));
     cur = std::move(next);
   }
+  tmark(s, "coarse");
   H.grid_complexity = (double)total_rows / (double)h.A.n;
   H.operator_complexity = (double)total_nnz / (double)h.A.nnz;
   if (H.precision >= 1) {
@@ -893,6 +1023,7 @@ void build_hierarchy(Handle& h) {
       sell_or_csr(L.R, 0, sell_lanes(L.R), H.precision >= 1, s);
     }
   }
+  tmark(s, "sell copies");
   H.ah.nslices = 0;
   H.ph.nslices = 0;
   H.rh.nslices = 0;

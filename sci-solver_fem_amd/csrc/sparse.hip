@@ -22,8 +22,8 @@ namespace fem {
 namespace {
 
 constexpr int kSymBlock = 64;
-constexpr int kSymCap = 256;             // LDS hash slots per thread (power of two)
-constexpr int kSymLimit = kSymCap * 3 / 4;
+constexpr int kSymCap = 256;             // LDS hash slots per thread (power of two), long product rows
+constexpr int kSymCapSmall = 128;        // the same for products of ~100 or fewer terms per row
 constexpr int kNumBlock = 64;
 constexpr int kNumCap = 64;              // LDS accumulator slots per thread
 
@@ -111,7 +111,7 @@ __device__ __forceinline__ void compact_sort_large(int* tab, int64_t cap, int co
   }
 }
 
-template <bool kFill>
+template <bool kFill, int CAP>
 __global__ void __launch_bounds__(kSymBlock) sym_lds_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
                                                           const int32_t* __restrict__ l_ci,
                                                           const int32_t* __restrict__ r_rp,
@@ -119,19 +119,19 @@ __global__ void __launch_bounds__(kSymBlock) sym_lds_kernel(int64_t nrows, const
                                                           const int32_t* __restrict__ colmap, int drop_diag,
                                                           int32_t* __restrict__ cnt, const int32_t* __restrict__ c_rp,
                                                           int32_t* __restrict__ c_ci) {
-  __shared__ int lds[kSymBlock * kSymCap];
+  __shared__ int lds[kSymBlock * CAP];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   if (kFill && cnt[i] < 0) return;
-  int* tab = lds + threadIdx.x * kSymCap;
-  for (int t = 0; t < kSymCap; ++t) tab[t] = -1;
+  int* tab = lds + threadIdx.x * CAP;
+  for (int t = 0; t < CAP; ++t) tab[t] = -1;
   int count = 0;
-  bool ok = row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, kSymCap - 1, count, kSymLimit);
+  bool ok = row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, CAP - 1, count, CAP * 3 / 4);
   if (!kFill) {
     cnt[i] = ok ? count : -1;
     return;
   }
-  compact_sort_small(tab, kSymCap, count);
+  compact_sort_small(tab, CAP, count);
   int o = c_rp[i];
   for (int t = 0; t < count; ++t) c_ci[o + t] = tab[t];
 }
@@ -365,14 +365,46 @@ __global__ void sell_sort_keys_kernel(int64_t n, const int32_t* __restrict__ rp,
   idx[r] = (int32_t)r;
 }
 
-// Window bases of each slice: the distinct aligned blocks (col >> kSellWinBits)
-// its columns touch, in order of first appearance (rows in order, entries in
-// row order).  More than kSellWindows blocks, or no entry at all: not compact.
+// Window bases of each slice, first pass, a wavefront per slice (lane = row
+// position): when the slice's columns span fewer than kSellWindows aligned
+// blocks (col >> kSellWinBits) from its lowest one, those consecutive blocks
+// are its windows (compact = 1); otherwise compact = -1 and the second pass
+// decides.  A slice without entries is not compact (0).
+__global__ void sell_window_range_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
+                                         const int32_t* __restrict__ ci, const int32_t* __restrict__ srow,
+                                         int32_t* __restrict__ swin, int32_t* __restrict__ compact) {
+  const int64_t sl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (sl >= nslices) return;  // the whole wavefront
+  const int rows = 64 / g;
+  int bmin = 0x7fffffff, bmax = -1;
+  const int64_t p = sl * rows + lane;
+  if (lane < rows && p < n) {
+    const int64_t r = sell_row(srow, p);
+    for (int q = rp[r]; q < rp[r + 1]; ++q) {
+      const int b = ci[q] >> kSellWinBits;
+      bmin = min(bmin, b);
+      bmax = max(bmax, b);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, off, 64));
+    bmax = max(bmax, __shfl_xor(bmax, off, 64));
+  }
+  const bool fast = bmax >= 0 && bmax - bmin < kSellWindows;
+  if (lane < kSellWindows) swin[sl * kSellWindows + lane] = fast ? (bmin + lane) << kSellWinBits : -1;
+  if (lane == 0) compact[sl] = fast ? 1 : (bmax >= 0 ? -1 : 0);
+}
+
+// Second pass, a thread per slice left undecided: the distinct blocks in order
+// of first appearance (rows in order, entries in row order); more than
+// kSellWindows of them: not compact.
 __global__ void sell_window_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                    const int32_t* __restrict__ ci, const int32_t* __restrict__ srow,
                                    int32_t* __restrict__ swin, int32_t* __restrict__ compact) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nslices) return;
+  if (sl >= nslices || compact[sl] != -1) return;
   const int rows = 64 / g;
   int blk[kSellWindows];
   int nb = 0;
@@ -498,10 +530,24 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
     HIP_CHECK(hipStreamSynchronize(s));
     return;
   }
+  // hash table size from the average number of product terms per row: half
+  // the LDS per thread (twice the blocks per CU) when rows are short; rows
+  // that do not fit take the global-memory path either way
+  bool small = true;
+  if (r_rp && n > 0 && r_rows > 0) {
+    int32_t rnnz = 0;
+    HIP_CHECK(hipMemcpyAsync(&rnnz, r_rp + r_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    small = ((double)L.nnz / (double)n) * ((double)rnnz / (double)r_rows) <= 128.0;
+  }
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
   if (n > 0) {
-    sym_lds_kernel<false><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
-                                                                      drop_diag ? 1 : 0, cnt.get(), nullptr, nullptr);
+    if (small)
+      sym_lds_kernel<false, kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), nullptr, nullptr);
+    else
+      sym_lds_kernel<false, kSymCap><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), nullptr, nullptr);
     HIP_LAUNCH_CHECK();
   }
   DevBuf<int32_t> rows(n > 0 ? n : 1);
@@ -532,9 +578,12 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   C.nnz = nnz;
   C.ci.resize(nnz > 0 ? nnz : 1);
   if (n > 0) {
-    sym_lds_kernel<true><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
-                                                                     drop_diag ? 1 : 0, cnt.get(), C.rp.get(),
-                                                                     C.ci.get());
+    if (small)
+      sym_lds_kernel<true, kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), C.rp.get(), C.ci.get());
+    else
+      sym_lds_kernel<true, kSymCap><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), C.rp.get(), C.ci.get());
     HIP_LAUNCH_CHECK();
   }
   if (nover) {
@@ -600,6 +649,9 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
   A.nnz_compact = 0;
   if (ns > 0) {
     DevBuf<int32_t> cflag(ns), cscan(ns + 1);
+    sell_window_range_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow,
+                                                                    A.swin.get(), cflag.get());
+    HIP_LAUNCH_CHECK();
     sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow, A.swin.get(),
                                                          cflag.get());
     HIP_LAUNCH_CHECK();
