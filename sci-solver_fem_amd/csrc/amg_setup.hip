@@ -15,8 +15,6 @@
 // deterministic function of (graph, seed) and matches the oracle bit for bit.
 #include <algorithm>
 #include <chrono>
-#include <cstdio>
-#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -326,10 +324,6 @@ __global__ void sumsq_kernel(int64_t n, const double* __restrict__ v, double* __
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void scale_kernel(int64_t n, double a, double* __restrict__ v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] *= a;
-}
 
 __global__ void hash_vector_kernel(int64_t n, unsigned long long seed, double* __restrict__ v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,35 +335,53 @@ __global__ void hash_vector_kernel(int64_t n, unsigned long long seed, double* _
   v[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-double norm2(const double* v, int64_t n, hipStream_t s) {
-  const int nb = 256;
-  DevBuf<double> part(nb);
-  sumsq_kernel<<<nb, kNT, 0, s>>>(n, v, part.get());
-  HIP_LAUNCH_CHECK();
-  std::vector<double> h = part.to_host(s);
+// one block: nrm = sqrt(sum of the nb partials, summed in index order)
+__global__ void norm_finish_kernel(int nb, const double* __restrict__ part, double* __restrict__ nrm) {
+  if (threadIdx.x != 0) return;
   double t = 0.0;
-  for (double x : h) t += x;
-  return std::sqrt(t);
+  for (int b = 0; b < nb; ++b) t += part[b];
+  nrm[0] = sqrt(t);
 }
 
-// Spectral radius of D^-1 A by power iteration (upper part of the spectrum).
+// v *= 1 / nrm when nrm > 0 (a zero or NaN norm leaves v as it is)
+__global__ void scale_by_norm_kernel(int64_t n, const double* __restrict__ nrm, double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = nrm[0];
+  if (i < n && a > 0) v[i] *= 1.0 / a;
+}
+
+// Spectral radius of D^-1 A by power iteration (upper part of the spectrum),
+// on the device with one read-back at the end.
 double estimate_rho(const Csr& A, const double* dinv, hipStream_t s, int iters = 15) {
   const int64_t n = A.n;
-  DevBuf<double> v(n), w(n);
+  const int nb = 256;
+  DevBuf<double> v(n), w(n), part(nb), nrm(iters + 1);
   hash_vector_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 12345ull, v.get());
   HIP_LAUNCH_CHECK();
-  double rho = 1.0;
-  double nv = norm2(v.get(), n, s);
-  scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 1.0 / nv, v.get());
+  auto norm_into = [&](const double* x, double* out) {
+    sumsq_kernel<<<nb, kNT, 0, s>>>(n, x, part.get());
+    HIP_LAUNCH_CHECK();
+    norm_finish_kernel<<<1, 64, 0, s>>>(nb, part.get(), out);
+    HIP_LAUNCH_CHECK();
+  };
+  norm_into(v.get(), nrm.get());
+  scale_by_norm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nrm.get(), v.get());
+  HIP_LAUNCH_CHECK();
   for (int it = 0; it < iters; ++it) {
     spmv64(A, v.get(), w.get(), s);
     dinv_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, dinv, w.get());
     HIP_LAUNCH_CHECK();
-    double nw = norm2(w.get(), n, s);
-    rho = nw;
-    if (!(nw > 0)) break;
-    scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 1.0 / nw, w.get());
+    norm_into(w.get(), nrm.get() + 1 + it);
+    scale_by_norm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nrm.get() + 1 + it, w.get());
+    HIP_LAUNCH_CHECK();
     std::swap(v, w);
+  }
+  std::vector<double> h = nrm.to_host(s);
+  // the last norm, or the first that was not positive (the iteration stops there)
+  double rho = 1.0;
+  for (int it = 0; it < iters; ++it) {
+    rho = h[1 + it];
+    if (!(rho > 0)) break;
   }
   return rho;
 }
@@ -493,29 +505,6 @@ bool dense_inverse(std::vector<double>& a, int64_t n) {
   return true;
 }
 
-// Strength graph: j != i with |a_ij| >= theta sqrt(a_ii a_jj) (1/dinv = a_ii).
-template <bool kFill>
-__global__ void strength_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                const double* __restrict__ v, const double* __restrict__ dinv, double theta,
-                                int32_t* __restrict__ cnt, const int32_t* __restrict__ gp, int32_t* __restrict__ gj) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double di = dinv[i];
-  int c = 0;
-  int o = kFill ? gp[i] : 0;
-  for (int q = rp[i]; q < rp[i + 1]; ++q) {
-    int j = ci[q];
-    if (j == (int)i) continue;
-    // |a_ij|^2 >= theta^2 a_ii a_jj  <=>  |a_ij|^2 dinv_i dinv_j >= theta^2
-    double a = v[q];
-    if (a * a * di * dinv[j] >= theta * theta) {
-      if (kFill) gj[o + c] = j;
-      ++c;
-    }
-  }
-  if (!kFill) cnt[i] = c;
-}
-
 // Pattern of A without the diagonal (the aggregation graph on the full pattern).
 template <bool kFill>
 __global__ void offdiag_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -623,19 +612,60 @@ bool agg_pattern(const Csr& A, const int32_t* agg, int64_t nagg, Csr& P, hipStre
   return true;
 }
 
+// Strength graph: j != i with |a_ij| >= theta sqrt(a_ii a_jj) (1/dinv = a_ii).
+// kStrengthLanes lanes per row take its entries in groups (lane t the t-th of
+// a group); the fill pass keeps the row order through the group's ballot.
+constexpr int kStrengthLanes = 4;
+
+template <bool kFill>
+__global__ void strength_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                const double* __restrict__ v, const double* __restrict__ dinv, double theta,
+                                int32_t* __restrict__ cnt, const int32_t* __restrict__ gp, int32_t* __restrict__ gj) {
+  constexpr int G = kStrengthLanes;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int t = threadIdx.x & (G - 1);
+  const int gshift = (threadIdx.x & 63) & ~(G - 1);  // first lane of this row's group
+  if (i >= n) return;  // the whole group
+  const double di = dinv[i];
+  const int s0 = rp[i], e0 = rp[i + 1];
+  int c = 0;
+  int o = kFill ? gp[i] : 0;
+  for (int q0 = s0; q0 < e0; q0 += G) {
+    const int q = q0 + t;
+    bool pass = false;
+    int j = 0;
+    if (q < e0) {
+      j = ci[q];
+      // |a_ij|^2 >= theta^2 a_ii a_jj  <=>  |a_ij|^2 dinv_i dinv_j >= theta^2
+      const double a = v[q];
+      pass = j != (int)i && a * a * di * dinv[j] >= theta * theta;
+    }
+    const unsigned grp = (unsigned)(__ballot(pass) >> gshift) & ((1u << G) - 1u);
+    if (kFill && pass) gj[o + __popc(grp & ((1u << t) - 1u))] = j;
+    o += __popc(grp);
+    c += __popc(grp);
+  }
+  if (!kFill && t == 0) cnt[i] = c;
+}
+
 void strength_graph(const Csr& A, const double* dinv, double theta, Csr& G, hipStream_t s) {
   const int64_t n = A.n;
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
-  strength_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta,
-                                                          cnt.get(), nullptr, nullptr);
-  HIP_LAUNCH_CHECK();
+  const unsigned grid = grid_for(n * kStrengthLanes, 256);
+  if (n > 0) {
+    strength_kernel<false><<<grid, 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta, cnt.get(),
+                                                nullptr, nullptr);
+    HIP_LAUNCH_CHECK();
+  }
   G.n = G.m = n;
   G.rp.resize(n + 1);
   G.nnz = dev_exclusive_scan_i32(cnt.get(), G.rp.get(), n, s);
   G.ci.resize(G.nnz > 0 ? G.nnz : 1);
-  strength_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta,
-                                                         nullptr, G.rp.get(), G.ci.get());
-  HIP_LAUNCH_CHECK();
+  if (n > 0) {
+    strength_kernel<true><<<grid, 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), dinv, theta, nullptr,
+                                               G.rp.get(), G.ci.get());
+    HIP_LAUNCH_CHECK();
+  }
 }
 
 void to_f32(Csr& A, hipStream_t s) {
@@ -858,18 +888,8 @@ void build_half_level0(Handle& h) {
   sell_or_csr(H.rh, 0, sell_lanes(H.rh), true, s);
 }
 
-static void tmark(hipStream_t s, const char* what) {  // TUNE (temporary)
-  static auto last = std::chrono::steady_clock::now();
-  if (!getenv("FEM_SETUP_TIMES")) return;
-  (void)hipStreamSynchronize(s);
-  auto now = std::chrono::steady_clock::now();
-  fprintf(stderr, "[setup] %-28s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
-  last = now;
-}
-
 void build_hierarchy(Handle& h) {
   auto t0 = std::chrono::steady_clock::now();
-  tmark(h.stream, "start");
   ++h.gen;  // invalidates a captured PCG iteration
   hipStream_t s = h.stream;
   Hierarchy& H = h.H;
@@ -885,7 +905,8 @@ void build_hierarchy(Handle& h) {
   // level-0 graph: pattern of A without the diagonal
   Csr G;
   if (h.prm.strengthThreshold <= 0.0) offdiag_graph(h.A, G, s);
-    tmark(s, "graph pattern");
+  // the fine operator's sliced copy first: the power iteration below runs on it
+  sell_or_csr(h.A, 0, sell_lanes(h.A), H.precision >= 1, s);
 
   int64_t total_rows = 0, total_nnz = 0;
   std::unique_ptr<Level> cur(new Level());
@@ -907,18 +928,15 @@ void build_hierarchy(Handle& h) {
       if (hb) fail(FEM_ERR_NUMERIC, "level " + std::to_string(l) + " has " + std::to_string(hb) +
                                         " rows without a positive diagonal");
     }
-    tmark(s, "diag");
     const bool last = (n < h.prm.topSize) || (l + 1 >= h.prm.maxLevels) || n <= 1;
     int64_t nagg = 0;
     const double theta = h.prm.strengthThreshold;
     if (!last && theta > 0.0) strength_graph(A, L.dinv64.get(), theta, G, s);
-    tmark(s, "strength");
     if (!last) {
       L.agg.resize(n);
       unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
       aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
                       &nagg, s);
-      tmark(s, "aggregate");
       if (theta > 0.0 && 2 * nagg > n) {
         // The strength graph fell apart into small components (each left as
         // singletons after runt removal): coarsening would stall for many
@@ -956,7 +974,6 @@ void build_hierarchy(Handle& h) {
     }
     L.nc = nagg;
     L.rho = estimate_rho(A, L.dinv64.get(), s);
-    tmark(s, "rho");
     // prolongator pattern: columns agg(k) for k in row i of A
     if (!agg_pattern(A, L.agg.get(), nagg, L.P, s))
       spgemm_symbolic(A, nullptr, nullptr, 0, L.agg.get(), nagg, false, L.P, s);
@@ -965,19 +982,15 @@ void build_hierarchy(Handle& h) {
                                                         L.agg.get(), omega, L.P.rp.get(), L.P.ci.get(),
                                                         L.P.v64.get());
     HIP_LAUNCH_CHECK();
-    tmark(s, "P");
     csr_transpose(L.P, L.R, s);
-    tmark(s, "R");
     // A_c = R (A P)
     std::unique_ptr<Level> next(new Level());
     {
       Csr AP;
       spgemm_symbolic(A, L.P.rp.get(), L.P.ci.get(), L.P.n, nullptr, nagg, false, AP, s);
       spgemm_numeric(A, L.P, AP, s);
-    tmark(s, "AP");
       spgemm_symbolic(L.R, AP.rp.get(), AP.ci.get(), AP.n, nullptr, nagg, false, next->A, s);
       spgemm_numeric(L.R, AP, next->A, s);
-    tmark(s, "RAP");
     }
     // next graph: induced aggregate graph (aggregates sharing an edge); with a
     // strength threshold the next level builds its own strength graph instead
@@ -991,7 +1004,6 @@ void build_hierarchy(Handle& h) {
     H.levels.push_back(std::move(cur));
     cur = std::move(next);
   }
-  tmark(s, "coarse");
   H.grid_complexity = (double)total_rows / (double)h.A.n;
   H.operator_complexity = (double)total_nnz / (double)h.A.nnz;
   if (H.precision >= 1) {
@@ -1014,8 +1026,8 @@ void build_hierarchy(Handle& h) {
     Level& L = *H.levels[l];
     Csr& A = (l == 0) ? h.A : L.A;
     // operators stay in row order (the fused PCG and level-0 smoother kernels
-    // read the fine one with one lane per row, in row order)
-    sell_or_csr(A, 0, sell_lanes(A), H.precision >= 1, s);
+    // read the fine one with one lane per row, in row order; built above)
+    if (l > 0) sell_or_csr(A, 0, sell_lanes(A), H.precision >= 1, s);
     if ((int)l != H.coarse) {
       // prolongator rows (1 to ~10 entries) are sorted by length within windows
       // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted
@@ -1023,7 +1035,6 @@ void build_hierarchy(Handle& h) {
       sell_or_csr(L.R, 0, sell_lanes(L.R), H.precision >= 1, s);
     }
   }
-  tmark(s, "sell copies");
   H.ah.nslices = 0;
   H.ph.nslices = 0;
   H.rh.nslices = 0;
