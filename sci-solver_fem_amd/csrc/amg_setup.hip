@@ -837,9 +837,9 @@ int sell_lanes(const Csr& M) {
   return g;
 }
 
-void sell_or_csr(Csr& M, int64_t sort_window, int lanes, bool with_f32, hipStream_t s) {
+void sell_or_csr(Csr& M, int64_t sort_window, int lanes, bool with_f32, hipStream_t s, bool split_diag = false) {
   if (lanes > 0) {
-    build_sell(M, with_f32, s, sort_window, false, true, lanes);
+    build_sell(M, with_f32, s, sort_window, false, true, lanes, split_diag && lanes == 1);
     return;
   }
   M.nslices = 0;
@@ -851,6 +851,9 @@ void sell_or_csr(Csr& M, int64_t sort_window, int lanes, bool with_f32, hipStrea
   M.swin.release();
   M.s64.release();
   M.s32.release();
+  M.sd64.release();
+  M.sd32.release();
+  M.nnz_sdiag = 0;
 }
 
 }  // namespace
@@ -905,8 +908,11 @@ void build_hierarchy(Handle& h) {
   // level-0 graph: pattern of A without the diagonal
   Csr G;
   if (h.prm.strengthThreshold <= 0.0) offdiag_graph(h.A, G, s);
-  // the fine operator's sliced copy first: the power iteration below runs on it
-  sell_or_csr(h.A, 0, sell_lanes(h.A), H.precision >= 1, s);
+  // the fine operator's sliced copy first (the power iteration below runs on
+  // it), with its diagonal outside the slices: the diagonal term then needs
+  // no column code and no gather (x_i is read by the row anyway; measured
+  // 3-5 % per level-0 pass at 161^3)
+  sell_or_csr(h.A, 0, sell_lanes(h.A), H.precision >= 1, s, true);
 
   int64_t total_rows = 0, total_nnz = 0;
   std::unique_ptr<Level> cur(new Level());

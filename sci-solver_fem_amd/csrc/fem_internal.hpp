@@ -55,6 +55,12 @@ struct Csr {
   DevBuf<double> s64;   // padded FP64 values
   DevBuf<float> s32;    // padded FP32 values (mixed precision only)
   DevBuf<uint16_t> sh;  // padded FP16 values (bit patterns; precision 2, level 0 only)
+  // Diagonal kept outside the slices (the fine operator): the slices hold the
+  // off-diagonal entries and a row's diagonal term a_ii x_i is added from
+  // these (empty: the diagonal is in the slices)
+  DevBuf<double> sd64;
+  DevBuf<float> sd32;
+  int64_t nnz_sdiag = 0;  // diagonal entries held in sd64 / sd32
 };
 
 struct Level {
@@ -194,9 +200,10 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
 void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s);
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
 // (re)build the sliced-ELL copy of A from its CSR arrays (FP64, and FP32 when
-// with_f32) with `lanes` (1, 2, 4, 8, 16 or 32) lanes per row
+// with_f32) with `lanes` (1, 2, 4, 8, 16 or 32) lanes per row; split_diag
+// (square A, one lane per row, row order) keeps the diagonal outside the slices
 void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0, bool with_f16 = false,
-                bool with_f64 = true, int lanes = 1);
+                bool with_f64 = true, int lanes = 1, bool split_diag = false);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----

@@ -171,11 +171,23 @@ struct SellView {
   const uint16_t* sc16; // 16-bit column codes of compact slices
   const int32_t* swin;  // kSellWindows window bases per slice (-1: not compact)
   const int32_t* srow;  // row at each position (null: position = row)
+  const double* sd64;   // diagonal outside the slices (null: in the slices)
+  const float* sd32;
 };
 
 inline SellView sell_view(const Csr& A) {
-  return SellView{A.sp.get(), A.sc.get(), A.sc16.get(), A.swin.get(), A.srow.size() ? A.srow.get() : nullptr};
+  return SellView{A.sp.get(), A.sc.get(), A.sc16.get(), A.swin.get(), A.srow.size() ? A.srow.get() : nullptr,
+                  A.sd64.size() ? A.sd64.get() : nullptr, A.sd32.size() ? A.sd32.get() : nullptr};
 }
+
+// the diagonal held outside the slices, in the working precision (null when
+// the slices hold it)
+template <typename T>
+__device__ __forceinline__ const T* sell_diag(const SellView& S);
+template <>
+__device__ __forceinline__ const double* sell_diag<double>(const SellView& S) { return S.sd64; }
+template <>
+__device__ __forceinline__ const float* sell_diag<float>(const SellView& S) { return S.sd32; }
 
 // Window bases of a compact slice, read once per wavefront into scalar
 // registers (named members, not an array: the select tree below must stay
@@ -268,6 +280,7 @@ __global__ void __launch_bounds__(kNT) sell_kernel(int64_t n, int64_t nslices, S
   const int64_t pos = (int64_t)slice * (64 / G) + lane / G;
   if ((G == 1 || (lane & (G - 1)) == 0) && pos < n) {
     const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
+    if (const T* sd = sell_diag<T>(S)) acc += sd[row] * x[row];  // the diagonal term, last
     spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
   }
 }
@@ -454,6 +467,7 @@ __global__ void __launch_bounds__(kNT) sell_multi_kernel(int64_t n, int k, int64
   if (!on || pos >= n) return;
   const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
   const int64_t idx = row * k + c0;
+  if (const T* sd = sell_diag<T>(S)) acc += sd[row] * *(const vec*)(x + idx);  // the diagonal term, last
   if (MODE == kAssign) {
     *(vec*)(y + idx) = acc;
   } else if (MODE == kAssignScale) {

@@ -262,9 +262,10 @@ __global__ void __launch_bounds__(kNT) sell_smooth_out_kernel(int64_t n, int64_t
   const int lane = threadIdx.x & 63;
   double part = 0.0;
   if (slice < nslices) {
-    const float acc = sell_row_sum<float>(S, slice, lane, sv, x);
+    float acc = sell_row_sum<float>(S, slice, lane, sv, x);
     const int64_t row = (int64_t)slice * 64 + lane;
     if (row < n) {
+      if (S.sd32) acc += S.sd32[row] * x[row];  // the diagonal term, last
       float xn;
       if (MODE == kJacobi) {
         xn = x[row] + c1 * dinv[row] * (b[row] - acc);
@@ -424,9 +425,10 @@ __global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t ns
       __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * (kNT / 64) + (threadIdx.x >> 6)));
   double part = 0.0;
   if (slice < nslices) {
-    const double acc = sell_row_sum<double>(S, slice, lane, sv, p);
+    double acc = sell_row_sum<double>(S, slice, lane, sv, p);
     const int64_t row = (int64_t)slice * 64 + lane;
     if (row < n) {
+      if (S.sd64) acc += S.sd64[row] * p[row];  // the diagonal term, last
       q[row] = acc;
       part = acc * p[row];
     }
@@ -586,6 +588,7 @@ __global__ void __launch_bounds__(kNT) pcg_sell_multi_dot_kernel(int64_t n, int 
       const int32_t* cp = S.sc + off + rl;
       for (int j = 0; j < w; ++j) acc += vp[64 * j] * *(const vec*)(p + (int64_t)cp[64 * j] * k + c0);
     }
+    if (S.sd64) acc += S.sd64[pos] * *(const vec*)(p + pos * k + c0);  // the diagonal term, last
     *(vec*)(q + pos * k + c0) = acc;
     part += acc * *(const vec*)(p + pos * k + c0);
   }
@@ -714,7 +717,8 @@ template <typename T>
 double matrix_bytes(const Csr& A, int k) {
   const bool sell = k == 1 && A.nslices > 0;
   const double nc = sell ? (double)A.nnz_compact : 0.0;
-  return (double)A.nnz * sizeof(T) + 2.0 * nc + 4.0 * ((double)A.nnz - nc);
+  const double nd = sell ? (double)A.nnz_sdiag : 0.0;  // diagonal outside the slices: no column code
+  return (double)A.nnz * sizeof(T) + 2.0 * nc + 4.0 * ((double)A.nnz - nd - nc);
 }
 
 template <typename T>

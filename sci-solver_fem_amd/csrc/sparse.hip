@@ -496,6 +496,34 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, int g, const int32_
   }
 }
 
+// Off-diagonal part of a square CSR matrix (count pass, then fill) and its
+// diagonal (0 where a row has none).
+template <bool kFill>
+__global__ void split_diag_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                  const double* __restrict__ v, int32_t* __restrict__ cnt,
+                                  const int32_t* __restrict__ orp, int32_t* __restrict__ oci, double* __restrict__ ov,
+                                  double* __restrict__ diag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  const int o = kFill ? orp[i] : 0;
+  double dv = 0.0;
+  for (int q = rp[i]; q < rp[i + 1]; ++q) {
+    const int j = ci[q];
+    if (j == (int)i) {
+      dv = v[q];
+      continue;
+    }
+    if (kFill) {
+      oci[o + c] = j;
+      ov[o + c] = v[q];
+    }
+    ++c;
+  }
+  if (kFill) diag[i] = dv;
+  else cnt[i] = c;
+}
+
 int bits_for(int64_t v) {
   int b = 1;
   while (b < 63 && ((int64_t)1 << b) <= v) ++b;
@@ -611,9 +639,56 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
 }
 
 void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool with_f16, bool with_f64,
-                int lanes) {
+                int lanes, bool split_diag) {
   if (lanes < 1 || lanes > 32 || (lanes & (lanes - 1)) != 0)
     fail(FEM_ERR_ARG, "sliced ELL: 1, 2, 4, 8, 16 or 32 lanes per row");
+  A.sd64.release();
+  A.sd32.release();
+  A.nnz_sdiag = 0;
+  if (split_diag) {
+    if (A.n != A.m || lanes != 1 || sort_window > 0 || with_f16)
+      fail(FEM_ERR_ARG, "sliced ELL: the diagonal is split off square matrices in row order only");
+    // slices of the off-diagonal entries, then the diagonal beside them
+    Csr T;
+    T.n = T.m = A.n;
+    DevBuf<int32_t> cnt(A.n > 0 ? A.n : 1);
+    DevBuf<double> diag(A.n > 0 ? A.n : 1);
+    if (A.n > 0) {
+      split_diag_kernel<false><<<grid_for(A.n, 256), 256, 0, s>>>(A.n, A.rp.get(), A.ci.get(), A.v64.get(), cnt.get(),
+                                                                  nullptr, nullptr, nullptr, nullptr);
+      HIP_LAUNCH_CHECK();
+    }
+    T.rp.resize(A.n + 1);
+    T.nnz = dev_exclusive_scan_i32(cnt.get(), T.rp.get(), A.n, s);
+    T.ci.resize(T.nnz > 0 ? T.nnz : 1);
+    T.v64.resize(T.nnz > 0 ? T.nnz : 1);
+    if (A.n > 0) {
+      split_diag_kernel<true><<<grid_for(A.n, 256), 256, 0, s>>>(A.n, A.rp.get(), A.ci.get(), A.v64.get(), nullptr,
+                                                                 T.rp.get(), T.ci.get(), T.v64.get(), diag.get());
+      HIP_LAUNCH_CHECK();
+    }
+    build_sell(T, with_f32, s, 0, false, with_f64, 1, false);
+    A.nslices = T.nslices;
+    A.sell_g = 1;
+    A.sp = std::move(T.sp);
+    A.sc = std::move(T.sc);
+    A.sc16 = std::move(T.sc16);
+    A.swin = std::move(T.swin);
+    A.ncompact = T.ncompact;
+    A.nnz_compact = T.nnz_compact;
+    A.srow.release();
+    A.s64 = std::move(T.s64);
+    A.s32 = std::move(T.s32);
+    A.sh.release();
+    A.nnz_sdiag = A.nnz - T.nnz;
+    if (with_f32 && A.n > 0) {
+      A.sd32.resize(A.n);
+      dev_f64_to_f32(diag.get(), A.sd32.get(), A.n, s);
+    }
+    if (with_f64) A.sd64 = std::move(diag);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
   const int64_t rows = 64 / lanes;
   const int64_t ns = (A.n + rows - 1) / rows;
   A.nslices = ns;
