@@ -61,6 +61,14 @@ struct Csr {
   DevBuf<double> sd64;
   DevBuf<float> sd32;
   int64_t nnz_sdiag = 0;  // diagonal entries held in sd64 / sd32
+  // LDS staging of the gathered vector (one lane per row, row order): block b
+  // of lds_bs consecutive slices first copies x at its distinct columns
+  // lcols[lcs[b] .. lcs[b+1]) (ascending) into LDS; lc holds each stored
+  // entry's 16-bit index into that list (same layout as sc).  lds_bs == 0: none.
+  int lds_bs = 0;
+  int lds_maxc = 0;            // largest distinct-column count of a block
+  DevBuf<uint16_t> lc;
+  DevBuf<int32_t> lcs, lcols;
 };
 
 struct Level {
@@ -204,6 +212,10 @@ void csr_transpose(const Csr& P, Csr& R, hipStream_t s);
 // (square A, one lane per row, row order) keeps the diagonal outside the slices
 void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window = 0, bool with_f16 = false,
                 bool with_f64 = true, int lanes = 1, bool split_diag = false);
+// LDS staging lists for A's sliced copy (blocks of bs slices); leaves A
+// unstaged (lds_bs = 0) when a block holds more than 16384 stored entries or
+// more than 8192 distinct columns
+void build_lds(Csr& A, int bs, hipStream_t s);
 void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----

@@ -169,7 +169,11 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
              hipStream_t s) {
   if (A.n == 0) return;
   const T* val = vals_of<T>(A);
-  if (k == 1 && A.nslices > 0 && svals_of<T>(A)) {
+  if (k == 1 && A.nslices > 0 && svals_of<T>(A) && A.lds_bs > 0) {
+    const unsigned nb = (unsigned)((A.nslices + A.lds_bs - 1) / A.lds_bs);
+    sell_lds_kernel<T, MODE><<<nb, kNT, sizeof(T) * (size_t)A.lds_maxc, s>>>(
+        A.n, A.nslices, sell_view(A), lds_view(A), svals_of<T>(A), x, y, b, dinv, d, xn, c1, c2);
+  } else if (k == 1 && A.nslices > 0 && svals_of<T>(A)) {
     switch (A.sell_g) {
       case 1: launch_sell<T, MODE, 1>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
       case 2: launch_sell<T, MODE, 2>(A, x, y, b, dinv, d, xn, c1, c2, s); break;
@@ -277,6 +281,45 @@ __global__ void __launch_bounds__(kNT) sell_smooth_out_kernel(int64_t n, int64_t
       const double zv = (double)xn;
       z[row] = zv;
       part = r[row] * zv;
+    }
+  }
+  const double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// The same on the LDS-staged copy: a block per L.bs slices, r.z partials per block.
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_lds_out_kernel(int64_t n, int64_t nslices, SellView S, LdsView L,
+                                                         const float* __restrict__ sv, const float* __restrict__ x,
+                                                         const float* __restrict__ b, const float* __restrict__ dinv,
+                                                         const float* __restrict__ d, float c1, float c2,
+                                                         const double* __restrict__ r, double* __restrict__ z,
+                                                         double* __restrict__ partials) {
+  extern __shared__ unsigned char lds_raw[];
+  float* xs = reinterpret_cast<float*>(lds_raw);
+  __shared__ double s_buf[kNT];
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  lds_stage<float>(L, blk, x, xs);
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  for (int q = threadIdx.x >> 6; q < L.bs; q += kNT / 64) {
+    const int slice = __builtin_amdgcn_readfirstlane((int)(blk * L.bs + q));
+    if (slice >= nslices) break;
+    float acc = lds_row_sum<float>(S, L, slice, lane, sv, xs);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row < n) {
+      if (S.sd32) acc += S.sd32[row] * x[row];  // the diagonal term, last
+      float xn;
+      if (MODE == kJacobi) {
+        xn = x[row] + c1 * dinv[row] * (b[row] - acc);
+      } else {
+        const float rr = ntload(dinv + row) * (ntload(b + row) - acc);
+        const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * rr;
+        xn = x[row] + dn;
+      }
+      const double zv = (double)xn;
+      z[row] = zv;
+      part += r[row] * zv;
     }
   }
   const double tot = block_sum(part, s_buf);
@@ -431,6 +474,33 @@ __global__ void __launch_bounds__(kNT) pcg_sell_dot_kernel(int64_t n, int64_t ns
       if (S.sd64) acc += S.sd64[row] * p[row];  // the diagonal term, last
       q[row] = acc;
       part = acc * p[row];
+    }
+  }
+  double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// The same on the LDS-staged copy: a block per L.bs slices, p.q partials per block.
+__global__ void __launch_bounds__(kNT) pcg_sell_lds_dot_kernel(int64_t n, int64_t nslices, SellView S, LdsView L,
+                                                             const double* __restrict__ sv,
+                                                             const double* __restrict__ p, double* __restrict__ q,
+                                                             double* __restrict__ partials) {
+  extern __shared__ unsigned char lds_raw[];
+  double* xs = reinterpret_cast<double*>(lds_raw);
+  __shared__ double s_buf[kNT];
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  lds_stage<double>(L, blk, p, xs);
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  for (int w = threadIdx.x >> 6; w < L.bs; w += kNT / 64) {
+    const int slice = __builtin_amdgcn_readfirstlane((int)(blk * L.bs + w));
+    if (slice >= nslices) break;
+    double acc = lds_row_sum<double>(S, L, slice, lane, sv, xs);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row < n) {
+      if (S.sd64) acc += S.sd64[row] * p[row];  // the diagonal term, last
+      q[row] = acc;
+      part += acc * p[row];
     }
   }
   double tot = block_sum(part, s_buf);
@@ -716,8 +786,10 @@ Smoother smoother_for(const Handle& h, const Level& L) {
 template <typename T>
 double matrix_bytes(const Csr& A, int k) {
   const bool sell = k == 1 && A.nslices > 0;
-  const double nc = sell ? (double)A.nnz_compact : 0.0;
   const double nd = sell ? (double)A.nnz_sdiag : 0.0;  // diagonal outside the slices: no column code
+  if (sell && A.lds_bs > 0)  // 16-bit LDS indices, and each block's column list
+    return (double)A.nnz * sizeof(T) + 2.0 * ((double)A.nnz - nd) + 4.0 * (double)A.lcols.size();
+  const double nc = sell ? (double)A.nnz_compact : 0.0;
   return (double)A.nnz * sizeof(T) + 2.0 * nc + 4.0 * ((double)A.nnz - nd - nc);
 }
 
@@ -782,6 +854,23 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
     HIP_LAUNCH_CHECK();
   };
   auto last_step = [&](int mode, T* cur, double c1, double c2) {
+    if (A.lds_bs > 0) {
+      const unsigned nb = (unsigned)((A.nslices + A.lds_bs - 1) / A.lds_bs);
+      const size_t lds = sizeof(float) * (size_t)A.lds_maxc;
+      if (mode == kJacobi)
+        sell_lds_out_kernel<kJacobi><<<nb, kNT, lds, s>>>(A.n, A.nslices, sell_view(A), lds_view(A), A.s32.get(),
+                                                          (const float*)cur, (const float*)b, (const float*)dinv,
+                                                          (const float*)d, (float)c1, (float)c2, io->r64, io->z64,
+                                                          io->partials);
+      else
+        sell_lds_out_kernel<kCheb><<<nb, kNT, lds, s>>>(A.n, A.nslices, sell_view(A), lds_view(A), A.s32.get(),
+                                                        (const float*)cur, (const float*)b, (const float*)dinv,
+                                                        (const float*)d, (float)c1, (float)c2, io->r64, io->z64,
+                                                        io->partials);
+      HIP_LAUNCH_CHECK();
+      io->nb = nb;
+      return;
+    }
     const unsigned g = grid_for(A.nslices, kNT / 64);
     if (mode == kJacobi)
       sell_smooth_out_kernel<kJacobi><<<g, kNT, 0, s>>>(A.n, A.nslices, sell_view(A), A.s32.get(), (const float*)cur,
@@ -1283,6 +1372,14 @@ void launch_sell_dot(Handle& h) {
   const Csr& A = h.A;
   if (A.srow.size() || A.sell_g != 1)
     fail(FEM_ERR_STATE, "the operator's sliced-ELL copy must keep the row order with one lane per row");
+  if (A.lds_bs > 0) {
+    const unsigned nb = (unsigned)((A.nslices + A.lds_bs - 1) / A.lds_bs);
+    pcg_sell_lds_dot_kernel<<<nb, kNT, sizeof(double) * (size_t)A.lds_maxc, h.stream>>>(
+        A.n, A.nslices, sell_view(A), lds_view(A), A.s64.get(), h.kp.get(), h.kq.get(), h.partials.get());
+    HIP_LAUNCH_CHECK();
+    finish(h, (int)nb, 1, F_ALPHA, 0);
+    return;
+  }
   unsigned g = grid_for(A.nslices, kNT / 64);
   pcg_sell_dot_kernel<<<g, kNT, 0, h.stream>>>(A.n, A.nslices, sell_view(A), A.s64.get(), h.kp.get(), h.kq.get(),
                                                h.partials.get());

@@ -12,6 +12,8 @@
 // accumulates into the row (LDS when it fits, otherwise its global slice),
 // finding each column by binary search.  No float atomics: results are
 // bit-reproducible.
+#include <climits>
+
 #include "fem_internal.hpp"
 #include "half.hpp"
 
@@ -524,6 +526,98 @@ __global__ void split_diag_kernel(int64_t n, const int32_t* __restrict__ rp, con
   else cnt[i] = c;
 }
 
+// ---- LDS staging lists -----------------------------------------------------------
+
+constexpr int kLdsSortCap = 16384;  // stored entries of one block, sorted in LDS (64 KB)
+constexpr int kLdsMaxCols = 8192;   // distinct columns of one block staged at solve time
+constexpr int kLdsThreads = 256;
+
+// ascending bitonic sort of a[0..P), P a power of two, by the whole block
+__device__ void block_bitonic_sort(int32_t* a, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += kLdsThreads) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int x = a[i], y = a[l];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            a[i] = y;
+            a[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Block b (slices [b bs, (b+1) bs)): its stored columns sorted, duplicates
+// replaced by INT_MAX, sorted again, so keys[0..nu) are the distinct columns
+// ascending.  Count pass: cnt[b] = nu; fill pass: the list at lcs[b] and each
+// entry's index into it.  Blocks above kLdsSortCap entries set *over.
+template <bool kFill>
+__global__ void __launch_bounds__(kLdsThreads) lds_cols_kernel(int64_t nslices, int bs, const int32_t* __restrict__ sp,
+                                                             const int32_t* __restrict__ sc, int32_t* __restrict__ cnt,
+                                                             const int32_t* __restrict__ lcs,
+                                                             int32_t* __restrict__ lcols, uint16_t* __restrict__ lc,
+                                                             int* __restrict__ over) {
+  extern __shared__ int32_t keys[];  // the block's only LDS (64 KB at the cap)
+  const int64_t b = blockIdx.x;
+  const int64_t s0 = b * bs, s1 = min(nslices, s0 + (int64_t)bs);
+  const int e0 = sp[s0], ne = sp[s1] - e0;
+  if (ne > kLdsSortCap) {
+    if (threadIdx.x == 0) {
+      if (!kFill) cnt[b] = 0;
+      over[0] = 1;
+    }
+    return;
+  }
+  int P = 1;
+  while (P < ne) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += kLdsThreads) keys[t] = t < ne ? sc[e0 + t] : INT_MAX;
+  __syncthreads();
+  block_bitonic_sort(keys, P);
+  // a thread owns elements threadIdx.x + q * kLdsThreads (q < P / kLdsThreads <= 64)
+  uint64_t dup = 0;
+  for (int t = threadIdx.x, q = 0; t < P; t += kLdsThreads, ++q)
+    if (t > 0 && keys[t] == keys[t - 1]) dup |= 1ull << q;
+  __syncthreads();
+  for (int t = threadIdx.x, q = 0; t < P; t += kLdsThreads, ++q)
+    if ((dup >> q) & 1ull) keys[t] = INT_MAX;
+  __syncthreads();
+  block_bitonic_sort(keys, P);
+  // distinct count: the first INT_MAX (every column is below it)
+  int nu = 0;
+  {
+    int hi = P;
+    while (nu < hi) {
+      const int mid = (nu + hi) >> 1;
+      if (keys[mid] < INT_MAX) nu = mid + 1; else hi = mid;
+    }
+  }
+  if (!kFill) {
+    if (threadIdx.x == 0) cnt[b] = nu;
+    return;
+  }
+  const int o = lcs[b];
+  for (int t = threadIdx.x; t < nu; t += kLdsThreads) lcols[o + t] = keys[t];
+  for (int t = threadIdx.x; t < ne; t += kLdsThreads) {
+    const int v = sc[e0 + t];
+    int lo = 0, hi = nu;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (keys[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    lc[e0 + t] = (uint16_t)lo;
+  }
+}
+
+__global__ void max_i32_kernel(int64_t n, const int32_t* __restrict__ a, int* __restrict__ mx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(mx, a[i]);
+}
+
 int bits_for(int64_t v) {
   int b = 1;
   while (b < 63 && ((int64_t)1 << b) <= v) ++b;
@@ -645,6 +739,11 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
   A.sd64.release();
   A.sd32.release();
   A.nnz_sdiag = 0;
+  A.lds_bs = 0;
+  A.lds_maxc = 0;
+  A.lc.release();
+  A.lcs.release();
+  A.lcols.release();
   if (split_diag) {
     if (A.n != A.m || lanes != 1 || sort_window > 0 || with_f16)
       fail(FEM_ERR_ARG, "sliced ELL: the diagonal is split off square matrices in row order only");
@@ -744,6 +843,44 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
     HIP_LAUNCH_CHECK();
   }
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void build_lds(Csr& A, int bs, hipStream_t s) {
+  A.lds_bs = 0;
+  A.lds_maxc = 0;
+  A.lc.release();
+  A.lcs.release();
+  A.lcols.release();
+  if (A.nslices <= 0 || A.sell_g != 1 || A.srow.size() || bs <= 0) return;
+  const int64_t ns = A.nslices;
+  const int64_t nb = (ns + bs - 1) / bs;
+  DevBuf<int32_t> cnt(nb);
+  DevBuf<int> flag(2);
+  dev_fill_i32(flag.get(), 0, 2, s);
+  const size_t lds = sizeof(int32_t) * kLdsSortCap;
+  lds_cols_kernel<false><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), cnt.get(), nullptr,
+                                                               nullptr, nullptr, flag.get());
+  HIP_LAUNCH_CHECK();
+  max_i32_kernel<<<grid_for(nb, 256), 256, 0, s>>>(nb, cnt.get(), flag.get() + 1);
+  HIP_LAUNCH_CHECK();
+  int hf[2] = {0, 0};
+  HIP_CHECK(hipMemcpyAsync(hf, flag.get(), sizeof(hf), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (hf[0] || hf[1] > kLdsMaxCols) return;
+  DevBuf<int32_t> lcs(nb + 1);
+  const int64_t tot = dev_exclusive_scan_i32(cnt.get(), lcs.get(), nb, s);
+  DevBuf<int32_t> lcols(tot > 0 ? tot : 1);
+  const int64_t ent = (int64_t)A.sc.size();
+  DevBuf<uint16_t> lc(ent > 0 ? ent : 1);
+  lds_cols_kernel<true><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), nullptr, lcs.get(),
+                                                              lcols.get(), lc.get(), flag.get());
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipStreamSynchronize(s));
+  A.lc = std::move(lc);
+  A.lcs = std::move(lcs);
+  A.lcols = std::move(lcols);
+  A.lds_maxc = hf[1];
+  A.lds_bs = bs;
 }
 
 void csr_transpose(const Csr& P, Csr& R, hipStream_t s) {
