@@ -101,7 +101,7 @@ def cpu_baseline(fem, size: int):
             "converged": res["converged"], "rel_residual": res["rel_residual"]}
 
 
-DOMINANT_KERNEL = "sell_kernel<float, 4, 1>"  # level-0 Chebyshev step (FP32 sliced ELL, fused epilogue)
+DOMINANT_KERNEL = "sell_lds_kernel<float, 4>"  # level-0 Chebyshev step (FP32 LDS-staged sliced ELL, fused epilogue)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_level0_chebyshev_step.json"
 
 
@@ -309,7 +309,7 @@ def main():
                 traffic = tj["traffic_bytes"]
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "level-0 Chebyshev smoother step (FP32 sliced-ELL SpMV fused with residual and update)",
+                "kernel": "level-0 Chebyshev smoother step (FP32 LDS-staged sliced-ELL SpMV fused with residual and update)",
                 "launches": prof["launches"], "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": prof["bytes_per_launch"]}
     if pcg_prof and pcg_prof["launches"]:

@@ -528,9 +528,122 @@ __global__ void split_diag_kernel(int64_t n, const int32_t* __restrict__ rp, con
 
 // ---- LDS staging lists -----------------------------------------------------------
 
-constexpr int kLdsSortCap = 16384;  // stored entries of one block, sorted in LDS (64 KB)
+constexpr int kLdsBitWords = 8192;  // bitmap words of one block: a column range of 262144
+constexpr int kLdsHashBits = 13;
+constexpr int kLdsHash = 1 << kLdsHashBits;  // hash slots of a wider block
 constexpr int kLdsMaxCols = 8192;   // distinct columns of one block staged at solve time
 constexpr int kLdsThreads = 256;
+
+// Block b (slices [b bs, (b+1) bs)): lo[b] = its lowest stored column, nw[b] =
+// the 32-bit words that a bitmap of its column range needs.  flag[0] = max nw
+// up to kLdsBitWords; blocks above it (a Morton order's jumps) are appended to
+// wide[0..flag[2]) and set nw[b] = -1.
+__global__ void __launch_bounds__(kLdsThreads) lds_range_kernel(int64_t nslices, int bs, const int32_t* __restrict__ sp,
+                                                              const int32_t* __restrict__ sc, int32_t* __restrict__ lo,
+                                                              int32_t* __restrict__ nw, int* __restrict__ flag,
+                                                              int32_t* __restrict__ wide) {
+  __shared__ int red[2][kLdsThreads / 64];
+  const int64_t b = blockIdx.x;
+  const int64_t s0 = b * bs, s1 = min(nslices, s0 + (int64_t)bs);
+  const int e0 = sp[s0], e1 = sp[s1];
+  int mn = INT_MAX, mx = -1;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kLdsThreads) {
+    const int c = sc[e];
+    mn = min(mn, c);
+    mx = max(mx, c);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = min(mn, __shfl_xor(mn, off, 64));
+    mx = max(mx, __shfl_xor(mx, off, 64));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wave] = mn;
+    red[1][wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kLdsThreads / 64; ++w) {
+      mn = min(mn, red[0][w]);
+      mx = max(mx, red[1][w]);
+    }
+    const int words = mx >= mn ? ((mx - mn) >> 5) + 1 : 0;
+    lo[b] = mx >= mn ? mn : 0;
+    if (words <= kLdsBitWords) {
+      nw[b] = words;
+      atomicMax(flag, words);
+    } else {
+      nw[b] = -1;
+      wide[atomicAdd(flag + 2, 1)] = (int32_t)b;
+    }
+  }
+}
+
+// Block b marks its stored columns in an LDS bitmap over [lo[b], lo[b] + 32 nw[b]);
+// the exclusive prefix of the words' popcounts ranks each column among the
+// block's distinct columns, which come out ascending.  Count pass: cnt[b] =
+// distinct columns; fill pass: the list at lcs[b] and each entry's rank.
+template <bool kFill>
+__global__ void __launch_bounds__(kLdsThreads) lds_cols_kernel(int64_t nslices, int bs, const int32_t* __restrict__ sp,
+                                                             const int32_t* __restrict__ sc,
+                                                             const int32_t* __restrict__ lo,
+                                                             const int32_t* __restrict__ nwb, int32_t* __restrict__ cnt,
+                                                             const int32_t* __restrict__ lcs,
+                                                             int32_t* __restrict__ lcols, uint16_t* __restrict__ lc) {
+  extern __shared__ uint32_t bits[];  // nw words of bitmap, then nw word prefixes
+  __shared__ int wsum[kLdsThreads / 64];
+  const int64_t b = blockIdx.x;
+  const int64_t s0 = b * bs, s1 = min(nslices, s0 + (int64_t)bs);
+  const int e0 = sp[s0], e1 = sp[s1];
+  const int c0 = lo[b], nw = nwb[b];
+  if (nw < 0) return;  // a wide block: lds_hash_kernel
+  uint32_t* pre = bits + nw;
+  for (int w = threadIdx.x; w < nw; w += kLdsThreads) bits[w] = 0u;
+  __syncthreads();
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kLdsThreads) {
+    const int v = sc[e] - c0;
+    atomicOr(&bits[v >> 5], 1u << (v & 31));
+  }
+  __syncthreads();
+  // thread t owns words [w0, w1)
+  const int per = (nw + kLdsThreads - 1) / kLdsThreads;
+  const int w0 = min(nw, (int)threadIdx.x * per), w1 = min(nw, w0 + per);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int base = 0, total = 0;
+  for (int w = 0; w < kLdsThreads / 64; ++w) {
+    if (w < wave) base += wsum[w];
+    total += wsum[w];
+  }
+  if (!kFill) {
+    if (threadIdx.x == 0) cnt[b] = total;
+    return;
+  }
+  int r = base + incl - mine;
+  const int o = lcs[b];
+  for (int w = w0; w < w1; ++w) {
+    pre[w] = (uint32_t)r;
+    uint32_t m = bits[w];
+    while (m) {
+      lcols[o + r++] = c0 + (w << 5) + (__ffs(m) - 1);
+      m &= m - 1u;
+    }
+  }
+  __syncthreads();
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kLdsThreads) {
+    const int v = sc[e] - c0;
+    const int w = v >> 5;
+    lc[e] = (uint16_t)(pre[w] + __popc(bits[w] & ((1u << (v & 31)) - 1u)));
+  }
+}
 
 // ascending bitonic sort of a[0..P), P a power of two, by the whole block
 __device__ void block_bitonic_sort(int32_t* a, int P) {
@@ -552,64 +665,75 @@ __device__ void block_bitonic_sort(int32_t* a, int P) {
   }
 }
 
-// Block b (slices [b bs, (b+1) bs)): its stored columns sorted, duplicates
-// replaced by INT_MAX, sorted again, so keys[0..nu) are the distinct columns
-// ascending.  Count pass: cnt[b] = nu; fill pass: the list at lcs[b] and each
-// entry's index into it.  Blocks above kLdsSortCap entries set *over.
+// Wide block b = wide[blockIdx.x]: its stored columns go into an LDS hash set
+// (linear probing, at most kLdsHash probes), the distinct ones are compacted
+// and sorted, and each entry's index is found by binary search.  Count pass:
+// cnt[b] = distinct columns; fill pass: the list at lcs[b] and each entry's
+// index.  A set that overflows sets *over (and cnt[b] = 0).
 template <bool kFill>
-__global__ void __launch_bounds__(kLdsThreads) lds_cols_kernel(int64_t nslices, int bs, const int32_t* __restrict__ sp,
+__global__ void __launch_bounds__(kLdsThreads) lds_hash_kernel(int64_t nslices, int bs, const int32_t* __restrict__ wide,
+                                                             const int32_t* __restrict__ sp,
                                                              const int32_t* __restrict__ sc, int32_t* __restrict__ cnt,
                                                              const int32_t* __restrict__ lcs,
                                                              int32_t* __restrict__ lcols, uint16_t* __restrict__ lc,
                                                              int* __restrict__ over) {
-  extern __shared__ int32_t keys[];  // the block's only LDS (64 KB at the cap)
-  const int64_t b = blockIdx.x;
+  extern __shared__ int32_t sm[];  // kLdsHash slots, then kLdsMaxCols keys
+  int32_t* table = sm;
+  int32_t* keys = sm + kLdsHash;
+  __shared__ int nu_s, bad;
+  const int64_t b = wide[blockIdx.x];
   const int64_t s0 = b * bs, s1 = min(nslices, s0 + (int64_t)bs);
-  const int e0 = sp[s0], ne = sp[s1] - e0;
-  if (ne > kLdsSortCap) {
+  const int e0 = sp[s0], e1 = sp[s1];
+  for (int t = threadIdx.x; t < kLdsHash; t += kLdsThreads) table[t] = -1;
+  if (threadIdx.x == 0) {
+    nu_s = 0;
+    bad = 0;
+  }
+  __syncthreads();
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kLdsThreads) {
+    const int c = sc[e];
+    unsigned h = ((unsigned)c * 2654435761u) >> (32 - kLdsHashBits);
+    int probe = 0;
+    for (; probe < kLdsHash; ++probe) {
+      const int prev = atomicCAS(&table[h], -1, c);
+      if (prev == -1 || prev == c) break;
+      h = (h + 1u) & (unsigned)(kLdsHash - 1);
+    }
+    if (probe == kLdsHash) bad = 1;
+  }
+  __syncthreads();
+  if (bad) {
     if (threadIdx.x == 0) {
       if (!kFill) cnt[b] = 0;
       over[0] = 1;
     }
     return;
   }
-  int P = 1;
-  while (P < ne) P <<= 1;
-  for (int t = threadIdx.x; t < P; t += kLdsThreads) keys[t] = t < ne ? sc[e0 + t] : INT_MAX;
-  __syncthreads();
-  block_bitonic_sort(keys, P);
-  // a thread owns elements threadIdx.x + q * kLdsThreads (q < P / kLdsThreads <= 64)
-  uint64_t dup = 0;
-  for (int t = threadIdx.x, q = 0; t < P; t += kLdsThreads, ++q)
-    if (t > 0 && keys[t] == keys[t - 1]) dup |= 1ull << q;
-  __syncthreads();
-  for (int t = threadIdx.x, q = 0; t < P; t += kLdsThreads, ++q)
-    if ((dup >> q) & 1ull) keys[t] = INT_MAX;
-  __syncthreads();
-  block_bitonic_sort(keys, P);
-  // distinct count: the first INT_MAX (every column is below it)
-  int nu = 0;
-  {
-    int hi = P;
-    while (nu < hi) {
-      const int mid = (nu + hi) >> 1;
-      if (keys[mid] < INT_MAX) nu = mid + 1; else hi = mid;
-    }
+  for (int t = threadIdx.x; t < kLdsHash; t += kLdsThreads) {
+    const int c = table[t];
+    if (c >= 0) keys[atomicAdd(&nu_s, 1)] = c;
   }
+  __syncthreads();
+  const int nu = nu_s;
   if (!kFill) {
     if (threadIdx.x == 0) cnt[b] = nu;
     return;
   }
+  int P = 1;
+  while (P < nu) P <<= 1;
+  for (int t = nu + (int)threadIdx.x; t < P; t += kLdsThreads) keys[t] = INT_MAX;
+  __syncthreads();
+  block_bitonic_sort(keys, P);
   const int o = lcs[b];
   for (int t = threadIdx.x; t < nu; t += kLdsThreads) lcols[o + t] = keys[t];
-  for (int t = threadIdx.x; t < ne; t += kLdsThreads) {
-    const int v = sc[e0 + t];
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kLdsThreads) {
+    const int v = sc[e];
     int lo = 0, hi = nu;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (keys[mid] < v) lo = mid + 1; else hi = mid;
     }
-    lc[e0 + t] = (uint16_t)lo;
+    lc[e] = (uint16_t)lo;
   }
 }
 
@@ -854,32 +978,52 @@ void build_lds(Csr& A, int bs, hipStream_t s) {
   if (A.nslices <= 0 || A.sell_g != 1 || A.srow.size() || bs <= 0) return;
   const int64_t ns = A.nslices;
   const int64_t nb = (ns + bs - 1) / bs;
-  DevBuf<int32_t> cnt(nb);
-  DevBuf<int> flag(2);
-  dev_fill_i32(flag.get(), 0, 2, s);
-  const size_t lds = sizeof(int32_t) * kLdsSortCap;
-  lds_cols_kernel<false><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), cnt.get(), nullptr,
-                                                               nullptr, nullptr, flag.get());
+  DevBuf<int32_t> lo(nb), nw(nb), cnt(nb), wide(nb);
+  DevBuf<int> flag(4);  // max bitmap words, max distinct columns, wide blocks, sort overflow
+  dev_fill_i32(flag.get(), 0, 4, s);
+  lds_range_kernel<<<(unsigned)nb, kLdsThreads, 0, s>>>(ns, bs, A.sp.get(), A.sc.get(), lo.get(), nw.get(),
+                                                        flag.get(), wide.get());
   HIP_LAUNCH_CHECK();
-  max_i32_kernel<<<grid_for(nb, 256), 256, 0, s>>>(nb, cnt.get(), flag.get() + 1);
-  HIP_LAUNCH_CHECK();
-  int hf[2] = {0, 0};
+  int hf[4] = {0, 0, 0, 0};
   HIP_CHECK(hipMemcpyAsync(hf, flag.get(), sizeof(hf), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  if (hf[0] || hf[1] > kLdsMaxCols) return;
+  const int nwide = hf[2];
+  const size_t lds = 2 * sizeof(uint32_t) * (size_t)(hf[0] > 0 ? hf[0] : 1);
+  const size_t slds = sizeof(int32_t) * (kLdsHash + kLdsMaxCols);
+  lds_cols_kernel<false><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), lo.get(), nw.get(),
+                                                               cnt.get(), nullptr, nullptr, nullptr);
+  HIP_LAUNCH_CHECK();
+  if (nwide > 0) {
+    lds_hash_kernel<false><<<(unsigned)nwide, kLdsThreads, slds, s>>>(ns, bs, wide.get(), A.sp.get(), A.sc.get(),
+                                                                      cnt.get(), nullptr, nullptr, nullptr,
+                                                                      flag.get() + 3);
+    HIP_LAUNCH_CHECK();
+  }
+  max_i32_kernel<<<grid_for(nb, 256), 256, 0, s>>>(nb, cnt.get(), flag.get() + 1);
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipMemcpyAsync(hf, flag.get(), sizeof(hf), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  const int maxc = hf[1];
+  if (hf[3] || maxc > kLdsMaxCols) return;  // a block too large to stage
   DevBuf<int32_t> lcs(nb + 1);
   const int64_t tot = dev_exclusive_scan_i32(cnt.get(), lcs.get(), nb, s);
   DevBuf<int32_t> lcols(tot > 0 ? tot : 1);
   const int64_t ent = (int64_t)A.sc.size();
   DevBuf<uint16_t> lc(ent > 0 ? ent : 1);
-  lds_cols_kernel<true><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), nullptr, lcs.get(),
-                                                              lcols.get(), lc.get(), flag.get());
+  lds_cols_kernel<true><<<(unsigned)nb, kLdsThreads, lds, s>>>(ns, bs, A.sp.get(), A.sc.get(), lo.get(), nw.get(),
+                                                              nullptr, lcs.get(), lcols.get(), lc.get());
   HIP_LAUNCH_CHECK();
+  if (nwide > 0) {
+    lds_hash_kernel<true><<<(unsigned)nwide, kLdsThreads, slds, s>>>(ns, bs, wide.get(), A.sp.get(), A.sc.get(),
+                                                                     nullptr, lcs.get(), lcols.get(), lc.get(),
+                                                                     flag.get() + 3);
+    HIP_LAUNCH_CHECK();
+  }
   HIP_CHECK(hipStreamSynchronize(s));
   A.lc = std::move(lc);
   A.lcs = std::move(lcs);
   A.lcols = std::move(lcols);
-  A.lds_maxc = hf[1];
+  A.lds_maxc = maxc;
   A.lds_bs = bs;
 }
 
