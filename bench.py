@@ -101,6 +101,14 @@ def cpu_baseline(fem, size: int):
             "converged": res["converged"], "rel_residual": res["rel_residual"]}
 
 
+def cheb_step_bytes(n, nnz):
+    """Algorithmic bytes of one FP32 Chebyshev smoother step on an n-row matrix
+    with nnz stored entries: CSR SpMV nnz (v + 4) + 4 (n + 1) + n 2 s, plus
+    n (v + s) for D^-1 and b and 2 n s for the direction read and written
+    (v = s = 4 bytes)."""
+    return int(nnz * 8 + 4 * (n + 1) + n * 8 + n * 8 + n * 8)
+
+
 DOMINANT_KERNEL = "sell_lds_kernel<float, 4>"  # level-0 Chebyshev step (FP32 LDS-staged sliced ELL, fused epilogue)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_level0_chebyshev_step.json"
 
@@ -143,9 +151,11 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib, 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         # bench.py --pmc-summary FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_MIB ALGO_BYTES "command"
+        #          STORED_BYTES
         f, w, cf, cw, mib, algo, cmd = sys.argv[2:9]
         res = pmc_summary(f, w, cf, cw, float(mib), cmd)
         res["algorithmic_bytes"] = int(algo)
+        res["stored_bytes"] = int(sys.argv[9])
         TRAFFIC_FILE.write_text(json.dumps(res, indent=1) + "\n")
         print(json.dumps(res, indent=1))
         return
@@ -214,6 +224,7 @@ def main():
     info = infos[-1]
     log(f"[bench] mixed: {dt * 1e3:.1f} ms/solve, {info['iterations']} iterations, res {info['rel_residual']:.2e}")
     value = n / dt
+    nnz0 = s.nnz()
     out["mixed"] = dict(ms=dt * 1e3, iterations=info["iterations"], rel_residual=info["rel_residual"],
                         levels=info["num_levels"], operator_complexity=info["operator_complexity"],
                         grid_complexity=info["grid_complexity"], setup_s=t_setup, assembly_s=t_asm,
@@ -299,19 +310,26 @@ def main():
     roof = None
     if prof and prof["launches"]:
         avg_ms = prof["total_ms"] / prof["launches"]
-        ach = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        # algorithmic bytes of one level-0 Chebyshev step: the CSR SpMV figure
+        # nnz (4 + 4) + 4 (n + 1) + 2 n 4 (x read, x_new written) plus the sweep's
+        # D^-1 and b reads and the Chebyshev direction d read and written (4 n 4)
+        algo = cheb_step_bytes(n, nnz0)
+        ach = algo / (avg_ms * 1e-3) / 1e9
         # HBM bytes per launch measured by rocprofv3 counters (committed under
         # profiles/); used only when it was measured on this same workload
         traffic = None
         if TRAFFIC_FILE.exists():
             tj = json.loads(TRAFFIC_FILE.read_text())
-            if int(tj.get("algorithmic_bytes", -1)) == int(prof["bytes_per_launch"]):
+            if int(tj.get("algorithmic_bytes", -1)) == algo and \
+                    int(tj.get("stored_bytes", -1)) == int(prof["bytes_per_launch"]):
                 traffic = tj["traffic_bytes"]
+        stored = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "level-0 Chebyshev smoother step (FP32 LDS-staged sliced-ELL SpMV fused with residual and update)",
-                "launches": prof["launches"], "avg_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": prof["bytes_per_launch"]}
+                "launches": prof["launches"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": algo,
+                "stored_bytes_per_launch": int(prof["bytes_per_launch"]), "stored_GBs": round(stored, 1),
+                "stored_frac": round(stored / HBM_PEAK_GBS, 4)}
     if pcg_prof and pcg_prof["launches"]:
         avg = pcg_prof["total_ms"] / pcg_prof["launches"]
         out["pcg_spmv_fp64"] = {"avg_us": round(avg * 1e3, 2),

@@ -123,3 +123,23 @@ def test_write_vtk(fem, tmp_path):
     assert txt[4] == f"POINTS {xyz.shape[0]} float"
     assert f"CELLS {tets.shape[0]} {tets.shape[0] * 5}" in txt
     assert txt.count("10") == tets.shape[0]
+
+
+def test_bench_algorithmic_bytes_of_the_chebyshev_step():
+    """The bench line prices a level-0 Chebyshev step as a CSR SpMV (4-byte values
+    and columns, x read and x_new written) plus D^-1, b and the direction d read
+    and written.  At the 161^3 cube the SpMV part is the 545.9 MB quoted for an
+    FP32 CSR SpMV there, and the nnz is the Kuhn-mesh closed form."""
+    import sys
+    sys.path.insert(0, str(ROOT))
+    import bench
+    N = 161
+    n = N ** 3
+    # P1 pattern of the Kuhn split: each interior vertex couples to itself and
+    # 14 neighbours; edges = 3 N^2 (N-1) axis + 3 N (N-1)^2 face + (N-1)^3 body
+    edges = 3 * N * N * (N - 1) + 3 * N * (N - 1) ** 2 + (N - 1) ** 3
+    nnz = n + 2 * edges
+    assert nnz == 61_979_041
+    spmv = nnz * 8 + 4 * (n + 1) + 2 * n * 4
+    assert round(spmv / 1e6, 1) == 545.9
+    assert bench.cheb_step_bytes(n, nnz) == spmv + 4 * n * 4 == 612_684_200
