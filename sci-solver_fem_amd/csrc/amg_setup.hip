@@ -821,12 +821,14 @@ namespace {
 
 // Lanes per row of a sliced-ELL copy, from the average row length: one lane
 // up to 24 entries (fine operators, prolongators), else the fewest lanes that
-// leave at most 20 entries to each (161^3: ~80-entry level-1 rows take 4
-// lanes, 18.4 us per Chebyshev step against 34.6 us for the 16-lane CSR
-// kernel; ~140-entry level-0 restriction rows take 8, 37.4 us against 62.4 us;
+// leave at most 20 entries to each (161^3 level 1 without LDS staging: its
+// ~46-entry rows take 4 lanes, 18.4 us per Chebyshev step against 34.6 us for
+// the 16-lane CSR kernel; ~140-entry level-0 restriction rows take 8, 37.4 us against 62.4 us;
 // 2x lanes, or rows sorted by length, measured slower).  0 = no sliced copy:
 // operators under 32768 rows with long rows have too few slices to fill the
 // chip and keep the CSR kernel (4,904 rows: 7.2 us sliced, 4.8 us CSR).
+constexpr int64_t kLdsCoarseRows = 65536;
+
 int sell_lanes(const Csr& M) {
   if (M.n == 0) return 0;
   const double avg = (double)M.nnz / (double)M.n;
@@ -1041,8 +1043,18 @@ void build_hierarchy(Handle& h) {
     Level& L = *H.levels[l];
     Csr& A = (l == 0) ? h.A : L.A;
     // operators stay in row order (the fused PCG and level-0 smoother kernels
-    // read the fine one with one lane per row, in row order; built above)
-    if (l > 0) sell_or_csr(A, 0, sell_lanes(A), H.precision >= 1, s);
+    // read the fine one with one lane per row, in row order; built above).
+    // Coarse operators of at least kLdsCoarseRows rows take one lane per row
+    // with LDS staging in blocks of 2 slices (161^3 level 1, 181,696 rows of
+    // ~46 entries: Chebyshev step 14.6 us, against 18.0 us with 8-slice
+    // blocks (355 blocks for 256 CUs), 15.5 us with 1-slice blocks and
+    // 18.4 us for the 4-lane copy without staging)
+    if (l > 0 && A.n >= kLdsCoarseRows) {
+      sell_or_csr(A, 0, 1, H.precision >= 1, s, true);
+      build_lds(A, 2, s);
+    } else if (l > 0) {
+      sell_or_csr(A, 0, sell_lanes(A), H.precision >= 1, s);
+    }
     if ((int)l != H.coarse) {
       // prolongator rows (1 to ~10 entries) are sorted by length within windows
       // of 1024 rows: 1.8x padding in row order at 81^3, ~1.0x sorted

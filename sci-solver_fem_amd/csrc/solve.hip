@@ -153,6 +153,9 @@ void launch_sell(const Csr& A, const T* x, T* y, const T* b, const T* dinv, T* d
 // 12 % slower, FP32 with 8 per lane 7 % slower)
 template <typename T>
 constexpr int sell_multi_cols() { return 4; }
+// longer average rows take the entry-split CSR kernel for k columns (161^3
+// level 1, ~45 entries per row: 45 us against ~69 us on the sliced copy)
+constexpr int64_t kMultiSellRow = 24;
 
 template <typename T, int MODE, int L>
 void launch_sell_multi(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T* d, T* xn, T c1, T c2,
@@ -190,7 +193,8 @@ void spmv_op(const Csr& A, int k, const T* x, T* y, const T* b, const T* dinv, T
       case 32: launch_vec<T, T, MODE, 32>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
       default: launch_vec<T, T, MODE, 64>(A, val, x, y, b, dinv, d, xn, c1, c2, s); break;
     }
-  } else if (k % sell_multi_cols<T>() == 0 && A.nslices > 0 && A.sell_g == 1 && svals_of<T>(A)) {
+  } else if (k % sell_multi_cols<T>() == 0 && A.nslices > 0 && A.sell_g == 1 && svals_of<T>(A) &&
+             A.nnz <= kMultiSellRow * A.n) {
     // column groups on the sliced-ELL copy
     switch (pow2_at_least(k / sell_multi_cols<T>())) {
       case 1: launch_sell_multi<T, MODE, 1>(A, k, x, y, b, dinv, d, xn, c1, c2, s); break;
