@@ -326,8 +326,9 @@ __global__ void __launch_bounds__(kNT) sell_lds_kernel(int64_t n, int64_t nslice
     const int slice = __builtin_amdgcn_readfirstlane((int)(blk * L.bs + q));
     if (slice >= nslices) break;
     T acc = lds_row_sum<T>(S, L, slice, lane, sv, xs);
-    const int64_t row = (int64_t)slice * 64 + lane;
-    if (row < n) {
+    const int64_t pos = (int64_t)slice * 64 + lane;
+    if (pos < n) {
+      const int64_t row = S.srow ? (int64_t)S.srow[pos] : pos;
       if (sd) acc += sd[row] * x[row];  // the diagonal term, last
       spmv_epilogue<MODE, T, T>(row, row, acc, y, x, b, dinv, d, xn, c1, c2);
     }

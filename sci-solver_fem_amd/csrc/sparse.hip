@@ -975,7 +975,7 @@ void build_lds(Csr& A, int bs, hipStream_t s) {
   A.lc.release();
   A.lcs.release();
   A.lcols.release();
-  if (A.nslices <= 0 || A.sell_g != 1 || A.srow.size() || bs <= 0) return;
+  if (A.nslices <= 0 || A.sell_g != 1 || bs <= 0) return;
   const int64_t ns = A.nslices;
   const int64_t nb = (ns + bs - 1) / bs;
   DevBuf<int32_t> lo(nb), nw(nb), cnt(nb), wide(nb);
