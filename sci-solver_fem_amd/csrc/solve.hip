@@ -549,7 +549,9 @@ template <int KP>
 __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, double* __restrict__ x,
                                                        double* __restrict__ r, const double* __restrict__ p,
                                                        const double* __restrict__ q, double* __restrict__ partials,
-                                                       const double* __restrict__ scal, float* __restrict__ r32) {
+                                                       const double* __restrict__ scal, float* __restrict__ r32,
+                                                       const float* __restrict__ dinv32, float* __restrict__ x0,
+                                                       float* __restrict__ d0, float c0) {
   __shared__ double s_buf[kNT];
   double part = 0.0;
   const int c = threadIdx.x & (KP - 1);
@@ -561,7 +563,15 @@ __global__ void __launch_bounds__(kNT) pcg_update_kernel(int64_t n, int k, doubl
       x[t] += a * p[t];
       double rv = r[t] - a * q[t];
       r[t] = rv;
-      if (r32) r32[t] = (float)rv;
+      if (r32) {
+        const float bv = (float)rv;
+        r32[t] = bv;
+        if (x0) {  // the level-0 smoother's first step from zero on fl32(r)
+          const float v = c0 * dinv32[row] * bv;
+          x0[t] = v;
+          if (d0) d0[t] = v;
+        }
+      }
       part += rv * rv;
     }
   }
@@ -1095,15 +1105,46 @@ struct PrecOut {
 // correction, sliced ELL: the FP64 <-> FP32 conversions are fused into the
 // first and last level-0 smoothing steps, and the last one also writes
 // per-block partials of r.z.
+struct First0 {
+  bool on = false;
+  float* b32 = nullptr;
+  const float* dinv = nullptr;
+  float* x0 = nullptr;
+  float* d0 = nullptr;
+  float c0 = 0.0f;
+};
+
 bool fused_level0(const Handle& h, int k, bool partials) {
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
   return partials && h.H.precision >= 1 && k == 1 && h.H.coarse > 0 && h.prm.preInnerIters >= 1 && nu_post >= 1 &&
          h.A.nslices > 0 && h.A.s32.size() > 0 && h.A.sell_g == 1;
 }
 
+// Outputs of the fused PCG update for the FP32 fused level-0 V-cycle: b32 =
+// fl32(r), and the smoother's first step from zero, x0 = c0 dinv b32 (and d0 =
+// x0 for Chebyshev)
+First0 first0_for(Handle& h, int k) {
+  First0 f;
+  if (!fused_level0(h, k, true) || h.H.precision != 1) return f;
+  Level& L0 = *h.H.levels[0];
+  const Smoother S = smoother_for(h, L0);
+  f.b32 = L0.b32.get();
+  f.dinv = L0.dinv32.get();
+  f.x0 = L0.x32.get();
+  if (S.type == 0) {
+    f.c0 = (float)S.omega;
+  } else {
+    f.c0 = (float)(1.0 / (0.5 * (S.lmax + S.lmin)));
+    f.d0 = L0.d32.get();
+  }
+  f.on = f.dinv != nullptr && f.x0 != nullptr;
+  return f;
+}
+
 // z = M r at level 0 (FP64 r); k columns.  z is written only on the fused
 // path; otherwise it is left where the V-cycle put it (no copy).  r32_ready:
-// the level-0 FP32 right-hand side already holds fl32(r).
+// the level-0 FP32 right-hand side already holds fl32(r) (on the fused path:
+// and the first smoothing step is taken, by the PCG update).
 PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* partials, bool r32_ready) {
   hipStream_t s = h.stream;
   Level& L0 = *h.H.levels[0];
@@ -1122,7 +1163,7 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
       io.b32 = L0.b32.get();
       io.z64 = z;
       io.partials = partials;
-      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io);
+      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io, r32_ready);
       if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
       o.nb = io.nb;
     }
@@ -1350,25 +1391,29 @@ void dot(Handle& h, int k, const double* a, const B* b, int mode, int slot) {
 }
 
 template <int KP>
-void launch_update(Handle& h, int k, double tol2, float* r32) {
+void launch_update(Handle& h, int k, double tol2, float* r32, const First0* f0) {
   const int64_t n = h.A.n;
   unsigned g = vec_grid(n, KP);
   pcg_update_kernel<KP><<<g, kNT, 0, h.stream>>>(n, k, h.kx.get(), h.kr.get(), h.kp.get(), h.kq.get(),
-                                                 h.partials.get(), h.scal.get(), r32);
+                                                 h.partials.get(), h.scal.get(), f0 ? f0->b32 : r32,
+                                                 f0 ? f0->dinv : nullptr, f0 ? f0->x0 : nullptr,
+                                                 f0 ? f0->d0 : nullptr, f0 ? f0->c0 : 0.0f);
   HIP_LAUNCH_CHECK();
   finish(h, (int)g, k, F_RR, 0, tol2);
 }
 
 // x += alpha p, r -= alpha q, |r|^2, and r32 = fl32(r) when r32 is given
-void update(Handle& h, int k, double tol2, float* r32) {
+// (with f0: fl32(r) into the level-0 right-hand side and the smoother's first
+// step from zero into its x and d)
+void update(Handle& h, int k, double tol2, float* r32, const First0* f0 = nullptr) {
   switch (pow2_at_least(k)) {
-    case 1: launch_update<1>(h, k, tol2, r32); break;
-    case 2: launch_update<2>(h, k, tol2, r32); break;
-    case 4: launch_update<4>(h, k, tol2, r32); break;
-    case 8: launch_update<8>(h, k, tol2, r32); break;
-    case 16: launch_update<16>(h, k, tol2, r32); break;
-    case 32: launch_update<32>(h, k, tol2, r32); break;
-    default: launch_update<64>(h, k, tol2, r32); break;
+    case 1: launch_update<1>(h, k, tol2, r32, f0); break;
+    case 2: launch_update<2>(h, k, tol2, r32, f0); break;
+    case 4: launch_update<4>(h, k, tol2, r32, f0); break;
+    case 8: launch_update<8>(h, k, tol2, r32, f0); break;
+    case 16: launch_update<16>(h, k, tol2, r32, f0); break;
+    case 32: launch_update<32>(h, k, tol2, r32, f0); break;
+    default: launch_update<64>(h, k, tol2, r32, f0); break;
   }
 }
 
@@ -1550,11 +1595,14 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     int restarts = 0;
     // the unfused mixed-precision V-cycle takes fl32(r), written by the update
     float* r32 = (h.H.precision >= 1 && !fused_level0(h, k, true)) ? h.H.levels[0]->b32.get() : nullptr;
+    // the fused one takes fl32(r) and its first smoothing step from the update
+    const First0 f0 = first0_for(h, k);
     while (any) {
       // z = M r and rz = r.z (fused into the last level-0 smoothing step when
       // possible); z stays where the preconditioner left it
       auto precondition_dot = [&](int mode, int slot, bool r32_ready) {
-        const PrecOut o = precondition_pcg(h, h.kr.get(), h.kz.get(), k, h.partials.get(), r32_ready && r32);
+        const PrecOut o =
+            precondition_pcg(h, h.kr.get(), h.kz.get(), k, h.partials.get(), r32_ready && (r32 || f0.on));
         if (o.nb) finish(h, (int)o.nb, k, mode, slot);
         else if (o.z32) dot(h, k, h.kr.get(), o.z32, mode, slot);
         else dot(h, k, h.kr.get(), o.z64, mode, slot);
@@ -1575,7 +1623,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
           pcg_p_kernel<double><<<grid_for(m, 256), 256, 0, s>>>(m, k, o.z64, h.kp.get(), h.scal.get());
         HIP_LAUNCH_CHECK();
         spmv_dot(h, k);           // q = A p, alpha
-        update(h, k, tol2, r32);  // x, r, rr, act
+        update(h, k, tol2, r32, f0.on ? &f0 : nullptr);  // x, r, rr, act
       };
       // Iteration n ends with the n-th |r|^2 finish of this solve; the host
       // reads its record while iteration n + 1 may already run: the next
@@ -1587,7 +1635,7 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       auto issue = [&](bool first) {
         if (first) {
           spmv_dot(h, k);
-          update(h, k, tol2, r32);
+          update(h, k, tol2, r32, f0.on ? &f0 : nullptr);
         } else {
           run_iteration(h, k, tol2, iteration);
         }
