@@ -1120,12 +1120,14 @@ bool fused_level0(const Handle& h, int k, bool partials) {
          h.A.nslices > 0 && h.A.s32.size() > 0 && h.A.sell_g == 1;
 }
 
-// Outputs of the fused PCG update for the FP32 fused level-0 V-cycle: b32 =
-// fl32(r), and the smoother's first step from zero, x0 = c0 dinv b32 (and d0 =
-// x0 for Chebyshev)
+// Outputs of the PCG update for an FP32 level-0 V-cycle: b32 = fl32(r), and
+// the smoother's first step from zero, x0 = c0 dinv b32 (and d0 = x0 for
+// Chebyshev).  Off for the FP16 level-0 smoother (its own first step) and for
+// hierarchies without level-0 smoothing.
 First0 first0_for(Handle& h, int k) {
   First0 f;
-  if (!fused_level0(h, k, true) || h.H.precision != 1) return f;
+  if (h.H.precision < 1 || h.H.coarse <= 0 || h.prm.preInnerIters < 1) return f;
+  if (h.H.precision == 2 && fused_level0(h, k, true) && h.H.ah.nslices > 0) return f;
   Level& L0 = *h.H.levels[0];
   const Smoother S = smoother_for(h, L0);
   f.b32 = L0.b32.get();
@@ -1143,9 +1145,10 @@ First0 first0_for(Handle& h, int k) {
 
 // z = M r at level 0 (FP64 r); k columns.  z is written only on the fused
 // path; otherwise it is left where the V-cycle put it (no copy).  r32_ready:
-// the level-0 FP32 right-hand side already holds fl32(r) (on the fused path:
-// and the first smoothing step is taken, by the PCG update).
-PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* partials, bool r32_ready) {
+// the level-0 FP32 right-hand side already holds fl32(r); first_ready: and
+// the level-0 smoother's first step from zero is in its x (and d).
+PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* partials, bool r32_ready,
+                         bool first_ready) {
   hipStream_t s = h.stream;
   Level& L0 = *h.H.levels[0];
   const int64_t m = h.A.n * k;
@@ -1163,7 +1166,7 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
       io.b32 = L0.b32.get();
       io.z64 = z;
       io.partials = partials;
-      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io, r32_ready);
+      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io, first_ready);
       if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
       o.nb = io.nb;
     }
@@ -1174,7 +1177,7 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
     convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
     HIP_LAUNCH_CHECK();
   }
-  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k);
+  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k, nullptr, r32_ready && first_ready);
   return o;
 }
 
@@ -1182,7 +1185,7 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
 unsigned precondition(Handle& h, const double* r, double* z, int k, double* partials = nullptr) {
   hipStream_t s = h.stream;
   const int64_t m = h.A.n * k;
-  const PrecOut o = precondition_pcg(h, r, z, k, partials, false);
+  const PrecOut o = precondition_pcg(h, r, z, k, partials, false, false);
   if (o.z32) {
     convert_kernel<float, double><<<grid_for(m, 256), 256, 0, s>>>(m, o.z32, z);
     HIP_LAUNCH_CHECK();
@@ -1595,14 +1598,15 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     int restarts = 0;
     // the unfused mixed-precision V-cycle takes fl32(r), written by the update
     float* r32 = (h.H.precision >= 1 && !fused_level0(h, k, true)) ? h.H.levels[0]->b32.get() : nullptr;
-    // the fused one takes fl32(r) and its first smoothing step from the update
+    // and the level-0 smoother's first step from zero (with fl32(r) on the
+    // fused path) comes from the update as well
     const First0 f0 = first0_for(h, k);
     while (any) {
       // z = M r and rz = r.z (fused into the last level-0 smoothing step when
       // possible); z stays where the preconditioner left it
       auto precondition_dot = [&](int mode, int slot, bool r32_ready) {
-        const PrecOut o =
-            precondition_pcg(h, h.kr.get(), h.kz.get(), k, h.partials.get(), r32_ready && (r32 || f0.on));
+        const PrecOut o = precondition_pcg(h, h.kr.get(), h.kz.get(), k, h.partials.get(),
+                                           r32_ready && (r32 || f0.on), r32_ready && f0.on);
         if (o.nb) finish(h, (int)o.nb, k, mode, slot);
         else if (o.z32) dot(h, k, h.kr.get(), o.z32, mode, slot);
         else dot(h, k, h.kr.get(), o.z64, mode, slot);
