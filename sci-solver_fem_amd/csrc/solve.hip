@@ -687,6 +687,102 @@ __global__ void __launch_bounds__(kNT) pcg_sell_multi_dot_kernel(int64_t n, int 
   }
 }
 
+// Last level-0 smoothing step for k columns (a multiple of 4) on the FP32
+// sliced copy, with per-block, per-column partials of r.z (r the FP64 Krylov
+// residual, z the step's FP32 output, which is written; d is not, the step
+// being the last).  Lanes and units as in pcg_sell_multi_dot_kernel (L lanes
+// per row, 4 columns each, one sweep front per XCD); per column z is what
+// sell_multi_kernel computes for the same step.
+template <int MODE, int L>
+__global__ void __launch_bounds__(kNT) sell_multi_out_kernel(int64_t n, int k, int64_t nslices, SellView S,
+                                                           const float* __restrict__ sv, const float* __restrict__ x,
+                                                           const float* __restrict__ b,
+                                                           const float* __restrict__ dinv,
+                                                           const float* __restrict__ d, float* __restrict__ xn,
+                                                           float c1, float c2, const double* __restrict__ r,
+                                                           double* __restrict__ partials) {
+  typedef float vec __attribute__((ext_vector_type(4)));
+  constexpr int RW = 64 / L;
+  __shared__ double s_buf[4 * kNT];
+  const int lane = threadIdx.x & 63;
+  const int c0 = (lane % L) * 4;
+  const bool on = c0 < k;
+  const XcdRows R = xcd_rows(nslices * L, kNT / 64);
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+  for (int64_t u = R.first + (threadIdx.x >> 6); u < R.hi; u += R.step) {
+    const int unit = __builtin_amdgcn_readfirstlane((int)u);
+    const int slice = unit / L;
+    const int rl = (unit % L) * RW + lane / L;
+    const int off = __builtin_amdgcn_readfirstlane(S.sp[slice]);
+    const int w = __builtin_amdgcn_readfirstlane((S.sp[slice + 1] - off) >> 6);
+    const int32_t* wp = S.swin + slice * kSellWindows;
+    const SellWin W{__builtin_amdgcn_readfirstlane(wp[0]), __builtin_amdgcn_readfirstlane(wp[1]),
+                    __builtin_amdgcn_readfirstlane(wp[2]), __builtin_amdgcn_readfirstlane(wp[3]),
+                    __builtin_amdgcn_readfirstlane(wp[4]), __builtin_amdgcn_readfirstlane(wp[5]),
+                    __builtin_amdgcn_readfirstlane(wp[6]), __builtin_amdgcn_readfirstlane(wp[7])};
+    const int64_t pos = (int64_t)slice * 64 + rl;
+    if (!on || pos >= n) continue;
+    const float* vp = sv + off + rl;
+    vec acc = (vec)(0.0f);
+    if (W.b0 >= 0) {
+      const uint16_t* cp = S.sc16 + off + rl;
+      int q = 0;
+      for (; q + 4 <= w; q += 4) {
+        const int j0 = sell_col(cp[64 * q], W), j1 = sell_col(cp[64 * (q + 1)], W);
+        const int j2 = sell_col(cp[64 * (q + 2)], W), j3 = sell_col(cp[64 * (q + 3)], W);
+        const float v0 = vp[64 * q], v1 = vp[64 * (q + 1)], v2 = vp[64 * (q + 2)], v3 = vp[64 * (q + 3)];
+        const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+        const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+        acc += v0 * g0;
+        acc += v1 * g1;
+        acc += v2 * g2;
+        acc += v3 * g3;
+      }
+      for (; q < w; ++q) acc += vp[64 * q] * *(const vec*)(x + (int64_t)sell_col(cp[64 * q], W) * k + c0);
+    } else {
+      const int32_t* cp = S.sc + off + rl;
+      int q = 0;
+      for (; q + 4 <= w; q += 4) {
+        const int j0 = cp[64 * q], j1 = cp[64 * (q + 1)], j2 = cp[64 * (q + 2)], j3 = cp[64 * (q + 3)];
+        const float v0 = vp[64 * q], v1 = vp[64 * (q + 1)], v2 = vp[64 * (q + 2)], v3 = vp[64 * (q + 3)];
+        const vec g0 = *(const vec*)(x + (int64_t)j0 * k + c0), g1 = *(const vec*)(x + (int64_t)j1 * k + c0);
+        const vec g2 = *(const vec*)(x + (int64_t)j2 * k + c0), g3 = *(const vec*)(x + (int64_t)j3 * k + c0);
+        acc += v0 * g0;
+        acc += v1 * g1;
+        acc += v2 * g2;
+        acc += v3 * g3;
+      }
+      for (; q < w; ++q) acc += vp[64 * q] * *(const vec*)(x + (int64_t)cp[64 * q] * k + c0);
+    }
+    const int64_t idx = pos * k + c0;
+    if (S.sd32) acc += S.sd32[pos] * *(const vec*)(x + idx);  // the diagonal term, last
+    vec z;
+    if (MODE == kJacobi) {
+      z = *(const vec*)(x + idx) + (c1 * dinv[pos]) * (*(const vec*)(b + idx) - acc);
+    } else {  // kCheb
+      const vec rr = dinv[pos] * (*(const vec*)(b + idx) - acc);
+      const vec dn = (c1 != 0.0f ? c1 * *(const vec*)(d + idx) : (vec)(0.0f)) + c2 * rr;
+      z = *(const vec*)(x + idx) + dn;
+    }
+    *(vec*)(xn + idx) = z;
+    p0 += r[idx] * (double)z[0];
+    p1 += r[idx + 1] * (double)z[1];
+    p2 += r[idx + 2] * (double)z[2];
+    p3 += r[idx + 3] * (double)z[3];
+  }
+  s_buf[4 * threadIdx.x] = p0;
+  s_buf[4 * threadIdx.x + 1] = p1;
+  s_buf[4 * threadIdx.x + 2] = p2;
+  s_buf[4 * threadIdx.x + 3] = p3;
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    const int c = threadIdx.x;
+    double sum = 0.0;
+    for (int t = c / 4; t < kNT; t += L) sum += s_buf[4 * t + (c % 4)];
+    partials[(int64_t)blockIdx.x * k + c] = sum;
+  }
+}
+
 // partials of a.b per column
 template <int KP, typename B>
 __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const double* __restrict__ a,
@@ -833,9 +929,53 @@ bool fused_out(const Level0Io* io, const Csr& A, int k) {
          A.srow.size() == 0 && A.sell_g == 1;
 }
 
+// The last level-0 smoothing step for k > 1 columns with r.z partials
+// (io->r64 and io->partials, no io->z64) on the FP32 sliced copy
+template <typename T>
+bool multi_out(const Level0Io* io, const Csr& A, int k) {
+  return std::is_same<T, float>::value && io && io->r64 && io->partials && !io->z64 && k > 1 && k % 4 == 0 &&
+         k <= 64 && A.nslices > 0 && A.s32.size() > 0 && A.srow.size() == 0 && A.sell_g == 1;
+}
+
+// launches sell_multi_out_kernel; returns the partial blocks written
+template <int MODE>
+unsigned launch_multi_out(const Csr& A, int k, const float* x, const float* b, const float* dinv, const float* d,
+                          float* xn, float c1, float c2, const double* r, double* partials, hipStream_t s) {
+  const int L = pow2_at_least(k / 4);
+  int64_t g = (A.nslices * L + kNT / 64 - 1) / (kNT / 64);
+  if (g > 4096) g = 4096;
+  const SellView S = sell_view(A);
+  const float* sv = A.s32.get();
+  switch (L) {
+    case 1:
+      sell_multi_out_kernel<MODE, 1><<<(unsigned)g, kNT, 0, s>>>(A.n, k, A.nslices, S, sv, x, b, dinv, d, xn, c1, c2,
+                                                                 r, partials);
+      break;
+    case 2:
+      sell_multi_out_kernel<MODE, 2><<<(unsigned)g, kNT, 0, s>>>(A.n, k, A.nslices, S, sv, x, b, dinv, d, xn, c1, c2,
+                                                                 r, partials);
+      break;
+    case 4:
+      sell_multi_out_kernel<MODE, 4><<<(unsigned)g, kNT, 0, s>>>(A.n, k, A.nslices, S, sv, x, b, dinv, d, xn, c1, c2,
+                                                                 r, partials);
+      break;
+    case 8:
+      sell_multi_out_kernel<MODE, 8><<<(unsigned)g, kNT, 0, s>>>(A.n, k, A.nslices, S, sv, x, b, dinv, d, xn, c1, c2,
+                                                                 r, partials);
+      break;
+    default:
+      sell_multi_out_kernel<MODE, 16><<<(unsigned)g, kNT, 0, s>>>(A.n, k, A.nslices, S, sv, x, b, dinv, d, xn, c1,
+                                                                  c2, r, partials);
+      break;
+  }
+  HIP_LAUNCH_CHECK();
+  return (unsigned)g;
+}
+
 // nu steps of the smoother from x = 0 (zero == true) or from the given x.
 // Buffers x and t ping-pong; returns the buffer that holds the result (with
-// io->z64 set, the last step writes there instead and nullptr is returned).
+// io->z64 set, the last step writes there instead and nullptr is returned;
+// with multi_out, the last step also writes the r.z partials).
 // first_done: the first step from zero is already in x (and d), taken by the
 // restriction that produced b.
 template <typename T>
@@ -858,6 +998,7 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
   }
   const bool in64 = zero && io && io->r64 && std::is_same<T, float>::value && k == 1;
   const bool out64 = !zero && fused_out<T>(io, A, k);
+  const bool outk = !zero && multi_out<T>(io, A, k);
   auto first_step = [&](T* xo, T* dout, double c) {
     if (in64) {
       scale_in_kernel<<<grid_for(A.n, 256), 256, 0, s>>>(A.n, (const float*)dinv, io->r64, io->b32, (float*)xo,
@@ -912,6 +1053,13 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
         last_step(kJacobi, cur, S.omega, 0.0);
         return nullptr;
       }
+      if constexpr (std::is_same<T, float>::value) {
+        if (outk && step == nu - 1) {
+          io->nb = launch_multi_out<kJacobi>(A, k, cur, b, dinv, nullptr, nxt, (float)S.omega, 0.0f, io->r64,
+                                             io->partials, s);
+          return nxt;
+        }
+      }
       ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
       spmv_op<T, kJacobi>(A, k, cur, nullptr, b, dinv, nullptr, nxt, w, T(0), s);
       T* tmp = cur; cur = nxt; nxt = tmp;
@@ -941,6 +1089,12 @@ T* smooth(Handle& h, int l, const Csr& A, const T* b, T* x, T* t, T* d, int nu, 
       ProfScope ps(h, prof ? "smooth_l0_out" : "", out_bytes);
       last_step(kCheb, cur, c1, c2);
       return nullptr;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+      if (outk && step == nu - 1) {
+        io->nb = launch_multi_out<kCheb>(A, k, cur, b, dinv, d, nxt, (float)c1, (float)c2, io->r64, io->partials, s);
+        return nxt;
+      }
     }
     ProfScope ps(h, prof ? "smooth_l0" : "", sweep_bytes);
     spmv_op<T, kCheb>(A, k, cur, nullptr, b, dinv, d, nxt, (T)c1, (T)c2, s);
@@ -1177,7 +1331,13 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
     convert_kernel<double, float><<<grid_for(m, 256), 256, 0, s>>>(m, r, L0.b32.get());
     HIP_LAUNCH_CHECK();
   }
-  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k, nullptr, r32_ready && first_ready);
+  // k > 1: the last level-0 smoothing step also writes the r.z partials
+  // (when it runs on the sliced copy; o.nb stays 0 otherwise)
+  Level0Io io;
+  io.r64 = r;
+  io.partials = partials;
+  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k, (partials && k > 1) ? &io : nullptr, r32_ready && first_ready);
+  o.nb = io.nb;
   return o;
 }
 
