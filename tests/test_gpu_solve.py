@@ -232,19 +232,22 @@ def test_queued_ahead_iteration_leaves_solution_unchanged(fem, gpu, oracle, mesh
     s.close()
 
 
-@pytest.mark.parametrize("k", [3, 8, 16])
+@pytest.mark.parametrize("k,smoother", [(3, 1), (8, 1), (12, 1), (16, 1), (32, 1), (64, 1), (8, 0)])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batched_widths(fem, gpu, oracle, meshgen, k, precision):
+def test_batched_widths(fem, gpu, oracle, meshgen, k, smoother, precision):
     """k columns take the one-column-per-lane kernels (k = 3), the 4-column
     group kernels on the sliced-ELL copies (fine operator, prolongators; FP32
     and FP64, operator + dot included) or the CSR column-group kernels with
-    split entries (long coarse rows, restrictions); every column must match
+    split entries (long coarse rows, restrictions); in mixed precision the
+    last level-0 smoothing step also writes the r.z partials (k a multiple of
+    4: 1, 2, 4, 8 or 16 lanes per row, k = 12 with idle lanes; Chebyshev and
+    Jacobi) and the PCG update takes the first one.  Every column must match
     its single-column solve."""
-    report(f"[gpu] batched k={k} precision={precision}")
+    report(f"[gpu] batched k={k} smoother={smoother} precision={precision}")
     xyz, conn, kappa, A, _ = _problem(meshgen, oracle, 11, 1.5, 1e-2, seed=8)
     s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
     s.set_params(solverType=1, tolerance=1e-9, maxIters=300, precision=precision, preInnerIters=2,
-                 postInnerIters=2, topSize=64)
+                 postInnerIters=2, topSize=64, smootherType=smoother)
     B = meshgen.gaussian_rhs(11, xyz.shape[0], k)
     X, info = s.solve(B)
     assert info["status"] == 0
