@@ -101,6 +101,20 @@ def cpu_baseline(fem, size: int):
             "converged": res["converged"], "rel_residual": res["rel_residual"]}
 
 
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations of one P1 tet element matrix: Jacobian (9), determinant (14),
+# cofactors and scaling (37), gradients of the 4 basis functions (6), the 10
+# stiffness entries (50), coefficient and volume scaling (11), the 10 mass
+# entries (10), rounded up
+ASSEMBLY_FLOP_PER_TET = 140
+
+
+def assembly_bytes(n, ne, nnz):
+    """Algorithmic bytes of 3D assembly: connectivity and kappa per tet, the
+    coordinates per vertex, one FP64 value per stored entry."""
+    return int(ne * (16 + 8) + n * 24 + nnz * 8)
+
+
 def cheb_step_bytes(n, nnz):
     """Algorithmic bytes of one FP32 Chebyshev smoother step on an n-row matrix
     with nnz stored entries: CSR SpMV nnz (v + 4) + 4 (n + 1) + n 2 s, plus
@@ -288,6 +302,23 @@ def main():
         out["adjoint"] = dict(ms=dta * 1e3, ratio_to_forward=dta / dt, iterations=ainfo["iterations"],
                               ratio_to_forward_incl_setup=dta / (t_setup + dt))
         log(f"[bench] adjoint: {dta * 1e3:.1f} ms ({dta / dt:.2f}x forward)")
+    # the assembly kernel alone (HIP events on the library stream): the same
+    # kappa assembled again; priced by bytes and by flops, its intensity being
+    # near the FP64 ridge
+    s.profile(True)
+    for _ in range(3):
+        s.update_kappa(kappa)
+    ap = s.profile_stats("assembly")
+    s.profile(False)
+    if ap["launches"]:
+        avg = ap["total_ms"] / ap["launches"] * 1e-3
+        abytes = assembly_bytes(n, ne, nnz0)
+        aflop = ASSEMBLY_FLOP_PER_TET * ne
+        out["assembly_kernel"] = dict(avg_us=round(avg * 1e6, 1), bytes=abytes, achieved_GBs=round(abytes / avg / 1e9, 1),
+                                      frac_hbm=round(abytes / avg / 1e9 / HBM_PEAK_GBS, 4), flop=aflop,
+                                      achieved_TFLOPs=round(aflop / avg / 1e12, 2),
+                                      frac_fp64=round(aflop / avg / 1e12 / FP64_PEAK_TFLOPS, 4))
+        log(f"[bench] assembly kernel: {avg * 1e3:.2f} ms, {abytes / avg / 1e9:.0f} GB/s, {aflop / avg / 1e12:.2f} TFLOP/s")
     s.close()
     del s
 

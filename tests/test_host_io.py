@@ -143,3 +143,14 @@ def test_bench_algorithmic_bytes_of_the_chebyshev_step():
     spmv = nnz * 8 + 4 * (n + 1) + 2 * n * 4
     assert round(spmv / 1e6, 1) == 545.9
     assert bench.cheb_step_bytes(n, nnz) == spmv + 4 * n * 4 == 612_684_200
+
+
+def test_bench_algorithmic_bytes_of_assembly():
+    """Assembly at the 161^3 cube: connectivity and kappa per tet (589.8 MB),
+    coordinates (100.2 MB) and one FP64 value per entry (495.8 MB)."""
+    import sys
+    sys.path.insert(0, str(ROOT))
+    import bench
+    n, ne, nnz = 161 ** 3, 6 * 160 ** 3, 61_979_041
+    assert ne == 24_576_000
+    assert round(bench.assembly_bytes(n, ne, nnz) / 1e6, 1) == 1185.8
