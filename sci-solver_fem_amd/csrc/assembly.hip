@@ -273,35 +273,65 @@ __device__ __forceinline__ int find_col(const int* cols, int len, int c) {
 
 // Row-owner assembly.  acc/cols point to LDS (short rows) or to the row's own
 // slice of the global CSR arrays (long rows).
+// The 4 entries that corner a of tet e (code = 4 e + a) adds to its row:
+// values and the corners' vertex ids
+__device__ __forceinline__ void element_row(int code, int dim, int npe, bool signed_mass, double mass_coeff,
+                                            double mden, const int4* __restrict__ conn,
+                                            const double4* __restrict__ X, const double* __restrict__ kappa,
+                                            double val[4], int vid[4]) {
+  const int e = code >> 2, a = code & 3;
+  const int4 c = conn[e];
+  Geo G;
+  element_geometry(X, c, dim, G);
+  const double kv = kappa[e] * G.meas;
+  const double mv = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * mden;
+  vid[0] = c.x;
+  vid[1] = c.y;
+  vid[2] = c.z;
+  vid[3] = c.w;
+  // gradient of corner a picked by selects, and the corner loop unrolled,
+  // so G stays in registers (a dynamic index would put it in scratch)
+  double ga[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    ga[d] = a == 0 ? G.g[0][d] : (a == 1 ? G.g[1][d] : (a == 2 ? G.g[2][d] : G.g[3][d]));
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const double sp = ga[0] * G.g[b][0] + ga[1] * G.g[b][1] + ga[2] * G.g[b][2];
+    val[b] = kv * sp + mv * (a == b ? 2.0 : 1.0);
+  }
+}
+
+// Row sums over the row's incident tets in ascending element order; two tets
+// are loaded and computed at a time (independent load chains), their entries
+// added in order, so the sums are those of a one-at-a-time walk
 __device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool signed_mass, double mass_coeff,
                                              const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
                                              const int4* __restrict__ conn, const double4* __restrict__ X,
                                              const double* __restrict__ kappa, const int* cols, int len,
                                              double* acc) {
   const double mden = (dim == 3) ? (1.0 / 20.0) : (1.0 / 12.0);
-  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1]; ++p) {
-    int code = v2e[p];
-    int e = code >> 2, a = code & 3;
-    int4 c = conn[e];
-    Geo G;
-    element_geometry(X, c, dim, G);
-    double kv = kappa[e] * G.meas;
-    double mv = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * mden;
-    const int v[4] = {c.x, c.y, c.z, c.w};
-    // gradient of corner a picked by selects, and the corner loop unrolled,
-    // so G and v stay in registers (a dynamic index would put them in scratch)
-    double ga[3];
+  int p = v2e_ptr[row];
+  const int pe = v2e_ptr[row + 1];
+  for (; p + 1 < pe; p += 2) {
+    double v0[4], v1[4];
+    int i0[4], i1[4];
+    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
+    element_row(v2e[p + 1], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v1, i1);
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-      ga[d] = a == 0 ? G.g[0][d] : (a == 1 ? G.g[1][d] : (a == 2 ? G.g[2][d] : G.g[3][d]));
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b < npe) {
-        double s = ga[0] * G.g[b][0] + ga[1] * G.g[b][1] + ga[2] * G.g[b][2];
-        double val = kv * s + mv * (a == b ? 2.0 : 1.0);
-        acc[find_col(cols, len, v[b])] += val;
-      }
-    }
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i1[b])] += v1[b];
+  }
+  if (p < pe) {
+    double v0[4];
+    int i0[4];
+    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
   }
 }
 
