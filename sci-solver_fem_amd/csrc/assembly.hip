@@ -302,9 +302,10 @@ __device__ __forceinline__ void element_row(int code, int dim, int npe, bool sig
   }
 }
 
-// Row sums over the row's incident tets in ascending element order; two tets
-// are loaded and computed at a time (independent load chains), their entries
-// added in order, so the sums are those of a one-at-a-time walk
+// Row sums over the row's incident tets in ascending element order; four
+// (then two) tets are loaded and computed at a time (independent load chains,
+// 161^3: 4.76 ms one at a time, 4.31 two, 4.15 four), their entries added in
+// order, so the sums are those of a one-at-a-time walk
 __device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool signed_mass, double mass_coeff,
                                              const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
                                              const int4* __restrict__ conn, const double4* __restrict__ X,
@@ -313,6 +314,26 @@ __device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool
   const double mden = (dim == 3) ? (1.0 / 20.0) : (1.0 / 12.0);
   int p = v2e_ptr[row];
   const int pe = v2e_ptr[row + 1];
+  for (; p + 3 < pe; p += 4) {
+    double v0[4], v1[4], v2[4], v3[4];
+    int i0[4], i1[4], i2[4], i3[4];
+    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
+    element_row(v2e[p + 1], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v1, i1);
+    element_row(v2e[p + 2], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v2, i2);
+    element_row(v2e[p + 3], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v3, i3);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i1[b])] += v1[b];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i2[b])] += v2[b];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < npe) acc[find_col(cols, len, i3[b])] += v3[b];
+  }
   for (; p + 1 < pe; p += 2) {
     double v0[4], v1[4];
     int i0[4], i1[4];
