@@ -45,12 +45,12 @@ def load_package():
 
 
 def solver_params(precision: int):
-    # tuned V-cycle (sweep over strength threshold 0-0.03 and Chebyshev degree
-    # 2-4 at 161^3): Chebyshev(3) pre/post on D^-1 A, aggregation on the
-    # strength graph at theta = 0.005, SA-AMG down to <= 256 rows
+    # tuned V-cycle (sweeps over strength threshold 0-0.03 and Chebyshev degree
+    # 2-5 at 161^3): Chebyshev(3) pre/post on D^-1 A, aggregation on the
+    # strength graph at theta = 0.01, SA-AMG down to <= 256 rows
     return dict(solverType=1, tolerance=1e-8, maxIters=500, precision=precision, smootherType=1,
                 preInnerIters=3, postInnerIters=3, postRelaxes=1, topSize=256, aggregationRule=1,
-                strengthThreshold=0.005)
+                strengthThreshold=0.01)
 
 
 def time_solves(torch, s, b, steps, warmup, profile_family=None):
