@@ -76,7 +76,9 @@ typedef struct fem_params {
   int aggregationRule;    /* 0: first aggregated neighbour in index order (the
                              reference kernel's behaviour), 1: most-adjacent
                              aggregate, ties to the lowest aggregate id       */
-  int coarseSolve;        /* 0: FP64 Cholesky-based dense inverse on device   */
+  int coarseSolve;        /* 0: dense FP64 inverse of the coarsest level
+                             (Cholesky, computed on the host at setup),
+                             applied on the device in FP64 in every precision */
   unsigned long long seed;/* seed of the randomized MIS (reference: time())  */
   double strengthThreshold;/* 0: aggregate the full pattern at level 0 and the
                              induced aggregate graph below (the reference);
@@ -204,11 +206,30 @@ int fem_get_level_prolongator(const fem_handle* h, int lvl, int64_t* nnz,
 /* y = M x for level lvl's operator (which = 0, n_lvl x n_lvl), prolongator
  * (1, n_lvl x n_lvl+1) or restriction (2, n_lvl+1 x n_lvl) through the
  * one-column kernels the V-cycle launches, in the precision it applies them
- * (FP32 values and vectors in precisions 1 and 2, FP64 in 0); host arrays in
- * internal numbering.  No reference counterpart: a test and diagnostic hook
- * beside the level exports above (the reference applies these matrices inside
- * AMG_Level::cycle, src/core/cuda/amg_level.cu:22-128). */
+ * (FP32 values and vectors in precisions 1 and 2, FP64 in 0); which = 3 on the
+ * coarsest level: y = A_c^-1 x, the direct solve the V-cycle applies there
+ * (FP64 inverse and FP64 sums in every precision, x and y rounded to the
+ * level's vector type; replaces the host LU of src/core/cuda/amg.cu:101-107,
+ * amg_level.cu:25-31).  Host arrays in internal numbering.  A test and
+ * diagnostic hook beside the level exports above (the reference applies these
+ * matrices inside AMG_Level::cycle, src/core/cuda/amg_level.cu:22-128). */
 int fem_apply_level(fem_handle* h, int lvl, int which, const double* x, double* y);
+/* Smoother data of level lvl (not the coarsest): rho, the power-iteration
+ * estimate of rho(D^-1 A); the Chebyshev interval [lmin, lmax] =
+ * [1.1 rho / 30, 1.1 rho] and the Jacobi weight omega = smootherWeight *
+ * 4 / (3 rho) derived from it.  Any output may be NULL.  Diagnostic hook; the
+ * reference's smoother has a fixed weight (src/core/cuda/gauss_seidel.cu:1279-1376). */
+int fem_get_level_smoother(const fem_handle* h, int lvl, double* rho, double* lmin, double* lmax,
+                           double* omega);
+/* One smoothing step on level lvl through the kernel the V-cycle launches, in
+ * the hierarchy's precision (precision 2: the FP32 kernels): mode 0 (Jacobi)
+ * y = x + c1 D^-1 (b - A x); mode 1 (Chebyshev) d = c1 d + c2 D^-1 (b - A x)
+ * and y = x + d.  Host arrays of n_lvl entries in internal numbering; d is
+ * read and overwritten.  Diagnostic hook for the smoother that replaces the
+ * reference's partition-local Jacobi sweeps (src/core/cuda/gauss_seidel.cu:
+ * 1279-1376, 3627-3736). */
+int fem_apply_smoother_step(fem_handle* h, int lvl, int mode, double c1, double c2, const double* x,
+                            const double* b, double* d, double* y);
 /* internal -> user vertex map (nv entries). */
 int fem_get_permutation(const fem_handle* h, int32_t* perm);
 
@@ -223,7 +244,11 @@ int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj,
 
 /* ---- profiling -------------------------------------------------------------*/
 /* Time each launch of the named kernel family with HIP events on the handle's
- * stream while enabled.  Families: "jacobi_l0", "spmv_l0", "pcg_spmv". */
+ * stream while enabled.  Families: "assembly", "element_grad", "pcg_spmv"
+ * (FP64 outer operator + p.q), "smooth_l0" (level-0 smoothing step),
+ * "smooth_l0_out" (fused last level-0 step), "resid_l0" (level-0 residual),
+ * "smooth_l0_h" / "smooth_l0_h_out" (the FP16 level-0 smoother of precision 2).
+ * Timing a launch makes the stream wait between launches (no graph replay). */
 int fem_profile_enable(fem_handle* h, int enable);
 int fem_profile_get(const fem_handle* h, const char* family, int64_t* launches,
                     double* total_ms, double* bytes_per_launch);

@@ -96,6 +96,8 @@ class FEMSolver:
             _handle = h
         object.__setattr__(self, "_h", _handle)
         object.__setattr__(self, "filename_", fname or "")
+        # bumped whenever the operator changes (update_kappa, a matrix file)
+        object.__setattr__(self, "_operator_gen", 0)
         p = FemParams()
         L.fem_get_params(self._h, C.byref(p))
         object.__setattr__(self, "_p", p)
@@ -166,6 +168,7 @@ class FEMSolver:
         return int(lib().fem_get_num_elements(self._h))
 
     def readMatlabSparseMatrix(self, filename: str) -> int:
+        object.__setattr__(self, "_operator_gen", self._operator_gen + 1)
         return int(lib().fem_read_matlab_sparse(self._h, filename.encode()))
 
     @staticmethod
@@ -208,10 +211,10 @@ class FEMSolver:
             st = L.fem_solve(self._h, bb.ctypes.data, x.ctypes.data, k, 0, C.byref(info))
         else:
             import torch
-            self._bind_torch_stream()
             bb = b.contiguous().to(torch.float64)
             x = torch.zeros_like(bb) if x0 is None else x0.detach().to(torch.float64).contiguous().clone()
             k = 1 if bb.dim() == 1 else bb.shape[1]
+            self._bind_torch_stream()
             st = L.fem_solve(self._h, _ptr(bb), _ptr(x), k, 1, C.byref(info))
         if st != _lib.FEM_OK and (raise_on_nonconvergence or st != _lib.FEM_ERR_NOT_CONVERGED):
             _check(self._h, st)
@@ -220,8 +223,20 @@ class FEMSolver:
         return x, d
 
     def _bind_torch_stream(self):
+        """Order the next library call after torch's pending work on its inputs.
+
+        Called after the input and output tensors are produced and right
+        before the C call.  On a torch side stream the library queues its work
+        on that stream (in order after every producer).  On torch's default
+        (null) stream the library keeps its own non-blocking stream, which
+        does not wait on the null stream, so the null stream is drained first.
+        Every library call returns after its own work is done, so torch's
+        later consumers of the outputs need no wait."""
         import torch
-        s = torch.cuda.current_stream().cuda_stream
+        cur = torch.cuda.current_stream()
+        s = cur.cuda_stream
+        if not s:
+            cur.synchronize()
         _check(self._h, lib().fem_set_stream(self._h, C.c_void_p(s)))
 
     def apply_operator(self, x):
@@ -232,10 +247,10 @@ class FEMSolver:
             _check(self._h, lib().fem_apply_operator(self._h, xx.ctypes.data, y.ctypes.data, k, 0))
             return y
         import torch
-        self._bind_torch_stream()
         xx = x.contiguous().to(torch.float64)
         y = torch.empty_like(xx)
         k = 1 if xx.dim() == 1 else xx.shape[1]
+        self._bind_torch_stream()
         _check(self._h, lib().fem_apply_operator(self._h, _ptr(xx), _ptr(y), k, 1))
         return y
 
@@ -250,6 +265,7 @@ class FEMSolver:
             k = np.ascontiguousarray(kappa, dtype=np.float64)
         else:
             k = kappa.detach().to("cpu", dtype=__import__("torch").float64).contiguous().numpy()
+        object.__setattr__(self, "_operator_gen", self._operator_gen + 1)
         _check(self._h, lib().fem_update_kappa(self._h, k.ctypes.data))
 
     def adjoint_grad(self, u, dLdu):
@@ -264,11 +280,11 @@ class FEMSolver:
                                         C.byref(info))
         else:
             import torch
-            self._bind_torch_stream()
             uu = u.contiguous().to(torch.float64)
             gg = dLdu.contiguous().to(torch.float64)
             g = torch.empty(self.num_elements(), dtype=torch.float64, device=uu.device)
             lam = torch.empty_like(uu)
+            self._bind_torch_stream()
             st = lib().fem_adjoint_grad(self._h, _ptr(uu), _ptr(gg), _ptr(g), _ptr(lam), 1, C.byref(info))
         if st not in (_lib.FEM_OK, _lib.FEM_ERR_NOT_CONVERGED):
             _check(self._h, st)
@@ -285,10 +301,10 @@ class FEMSolver:
             _check(self._h, lib().fem_element_grad(self._h, aa.ctypes.data, uu.ctypes.data, g.ctypes.data, 0))
             return g
         import torch
-        self._bind_torch_stream()
         aa = a.contiguous().to(torch.float64)
         uu = u.contiguous().to(torch.float64)
         g = torch.empty(self.num_elements(), dtype=torch.float64, device=aa.device)
+        self._bind_torch_stream()
         _check(self._h, lib().fem_element_grad(self._h, _ptr(aa), _ptr(uu), _ptr(g), 1))
         return g
 
@@ -342,13 +358,37 @@ class FEMSolver:
         _check(self._h, lib().fem_get_level_info(self._h, lvl, C.byref(n), None))
         if which in (1, 2):
             _check(self._h, lib().fem_get_level_info(self._h, lvl + 1, C.byref(nc), None))
-        rows, cols = {0: (n.value, n.value), 1: (n.value, nc.value), 2: (nc.value, n.value)}.get(which, (0, 0))
+        rows, cols = {0: (n.value, n.value), 1: (n.value, nc.value), 2: (nc.value, n.value),
+                      3: (n.value, n.value)}.get(which, (0, 0))
         xx = np.ascontiguousarray(x, dtype=np.float64)
         if xx.shape != (cols,):
             raise ValueError(f"x must have {cols} entries")
         y = np.empty(rows, dtype=np.float64)
         _check(self._h, lib().fem_apply_level(self._h, lvl, which, xx.ctypes.data, y.ctypes.data))
         return y
+
+    def level_smoother(self, lvl: int) -> dict:
+        """Smoother data of level `lvl`: rho(D^-1 A) estimate, Chebyshev
+        interval [lmin, lmax] and Jacobi weight omega."""
+        v = [C.c_double() for _ in range(4)]
+        _check(self._h, lib().fem_get_level_smoother(self._h, lvl, *[C.byref(t) for t in v]))
+        return dict(zip(("rho", "lmin", "lmax", "omega"), (t.value for t in v)))
+
+    def smoother_step(self, lvl: int, mode: int, c1: float, c2: float, x, b, d):
+        """One smoothing step through the V-cycle's kernel (internal numbering):
+        returns (y, d_new).  mode 0: y = x + c1 D^-1 (b - A x); mode 1:
+        d_new = c1 d + c2 D^-1 (b - A x), y = x + d_new."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        dd = np.array(d, dtype=np.float64, copy=True, order="C")
+        n = C.c_int64()
+        _check(self._h, lib().fem_get_level_info(self._h, lvl, C.byref(n), None))
+        if not (xx.shape == bb.shape == dd.shape == (n.value,)):
+            raise ValueError(f"x, b and d must be vectors of the level's {n.value} rows")
+        y = np.empty_like(xx)
+        _check(self._h, lib().fem_apply_smoother_step(self._h, lvl, mode, float(c1), float(c2), xx.ctypes.data,
+                                                      bb.ctypes.data, dd.ctypes.data, y.ctypes.data))
+        return y, dd
 
     def permutation(self) -> np.ndarray:
         p = np.empty(self.getMatrixRows(), dtype=np.int32)
@@ -472,6 +512,9 @@ def _torch_function():
         hierarchy; backward reuses that hierarchy (K is symmetric):
             lambda = K^-1 dL/du,  dL/dkappa_e = -|T_e| grad(lambda).grad(u),
             dL/db = lambda.
+        When the solver's operator changed after this node's forward (a later
+        forward with another kappa on the same solver), backward reassembles
+        K for this node's kappa first, so every node's adjoint uses its own K.
         """
 
         @staticmethod
@@ -479,14 +522,18 @@ def _torch_function():
             solver.update_kappa(kappa)
             u, info = solver.solve(b, raise_on_nonconvergence=True)
             ctx.solver = solver
-            ctx.save_for_backward(u)
+            ctx.operator_gen = solver._operator_gen
+            ctx.save_for_backward(u, kappa)
             ctx.info = info
             return u
 
         @staticmethod
         def backward(ctx, grad_u):
-            (u,) = ctx.saved_tensors
+            u, kappa = ctx.saved_tensors
             solver = ctx.solver
+            if solver._operator_gen != ctx.operator_gen:
+                solver.update_kappa(kappa)
+                ctx.operator_gen = solver._operator_gen
             lam, _ = solver.solve(grad_u.contiguous(), raise_on_nonconvergence=True)
             g = solver.element_grad(lam, u)
             return g, lam, None

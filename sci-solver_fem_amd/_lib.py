@@ -99,6 +99,8 @@ SIGNATURES = {
     "fem_apply_operator": (C.c_int, [_P, _P, _P, C.c_int, C.c_int]),
     "fem_apply_preconditioner": (C.c_int, [_P, _P, _P, C.c_int]),
     "fem_apply_level": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "fem_get_level_smoother": (C.c_int, [_P, C.c_int, _PD, _PD, _PD, _PD]),
+    "fem_apply_smoother_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double, _P, _P, _P, _P]),
     "fem_get_num_levels": (C.c_int, [_P]),
     "fem_get_level_info": (C.c_int, [_P, C.c_int, _PI64, _PI64]),
     "fem_get_level_matrix": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),

@@ -31,8 +31,7 @@ def flags(src: str):
          f"--offload-arch={ARCH}"]
     if src.endswith(".hip"):
         # keep packed FP32 instructions out of the device code
-        f += ["-x", "hip", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-              "-munsafe-fp-atomics"]
+        f += ["-x", "hip", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
     return f
 
 

@@ -909,7 +909,6 @@ void build_hierarchy(Handle& h) {
   Hierarchy& H = h.H;
   H.levels.clear();
   H.cinv64.release();
-  H.cinv32.release();
   H.precision = h.prm.precision;
   H.built = false;
   const int fine_depth = h.prm.randMisParameters % 100;
@@ -983,11 +982,8 @@ void build_hierarchy(Handle& h) {
       for (int64_t i = 0; i < n; ++i)
         for (int q = rp[i]; q < rp[i + 1]; ++q) dense[i * n + ci[q]] += v[q];
       if (!dense_inverse(dense, n)) fail(FEM_ERR_NUMERIC, "coarsest-level matrix is singular");
+      // applied in FP64 in every precision mode (coarse_kernel)
       H.cinv64.upload(dense.data(), n * n, s);
-      if (H.precision >= 1) {
-        H.cinv32.resize(n * n);
-        dev_f64_to_f32(H.cinv64.get(), H.cinv32.get(), n * n, s);
-      }
       H.coarse = l;
       H.nc = n;
       H.levels.push_back(std::move(cur));

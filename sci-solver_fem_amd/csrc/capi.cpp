@@ -41,6 +41,20 @@ int guarded(fem_handle* hh, F&& f) {
   }
 }
 
+// Host readers called outside guarded(): an exception (a failed host
+// allocation, a stream error) becomes the reader's own failure code.
+template <typename F>
+int reader_guarded(int fail_code, std::string& err, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    err = "host out of memory while reading the file";
+  } catch (const std::exception& e) {
+    err = std::string("error while reading the file: ") + e.what();
+  }
+  return fail_code;
+}
+
 Handle* H(fem_handle* h) { return reinterpret_cast<Handle*>(h); }
 const Handle* H(const fem_handle* h) { return reinterpret_cast<const Handle*>(h); }
 
@@ -248,15 +262,23 @@ int fem_create_from_files(const char* base, int is_tet, int verbose, fem_handle*
   *out = nullptr;
   MeshData m;
   std::string err;
-  int rc = is_tet ? read_tetgen(base, m, err) : read_trimesh(base, m, err);
+  int rc = reader_guarded(2, err, [&]() { return is_tet ? read_tetgen(base, m, err) : read_trimesh(base, m, err); });
   if (rc) {
     g_create_error = err;
     return FEM_ERR_IO;
   }
   const int npe = m.dim + 1;
   int64_t ne = (int64_t)m.conn.size() / npe;
-  std::vector<double> kappa(ne, 1.0);
-  for (int64_t e = 0; e < ne; ++e) kappa[e] = label_coefficient(m.labels[e]);
+  std::vector<double> kappa;
+  rc = reader_guarded(2, err, [&]() {
+    kappa.assign(ne, 1.0);
+    for (int64_t e = 0; e < ne; ++e) kappa[e] = label_coefficient(m.labels[e]);
+    return 0;
+  });
+  if (rc) {
+    g_create_error = err;
+    return FEM_ERR_CAPACITY;
+  }
   return fem_create_from_arrays(m.dim, (int64_t)m.xyz.size() / 3, m.xyz.data(), ne, m.conn.data(), kappa.data(), 1.0,
                                 m.dim == 2 ? 1 : 0, verbose, out);
 }
@@ -344,7 +366,7 @@ int fem_read_matlab_sparse(fem_handle* hh, const char* filename) {
   Handle& h = *H(hh);
   SparseCoo A;
   std::string err;
-  int rc = read_matlab_sparse(filename, A, err);
+  int rc = reader_guarded(1, err, [&]() { return read_matlab_sparse(filename, A, err); });
   if (rc != 0) {
     h.err = err;
     return rc;
@@ -531,8 +553,38 @@ int fem_apply_level(fem_handle* hh, int lvl, int which, const double* x, double*
     const Csr& M = level_matrix_of(h, lvl, which);
     DevBuf<double> dx(M.m > 0 ? M.m : 1), dy(M.n > 0 ? M.n : 1);
     dx.upload(x, M.m, h.stream);
+    if (which < 0 || which > 3) fail(FEM_ERR_ARG, "which: 0 operator, 1 prolongator, 2 restriction, 3 coarse solve");
     apply_level_int(h, lvl, which, dx.get(), dy.get());
     dy.download(y, M.n, h.stream);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    return (int)FEM_OK;
+  });
+}
+
+int fem_get_level_smoother(const fem_handle* hc, int lvl, double* rho, double* lmin, double* lmax, double* omega) {
+  if (!hc) return FEM_ERR_ARG;
+  fem_handle* hh = const_cast<fem_handle*>(hc);
+  return guarded(hh, [&]() {
+    level_smoother_info(*H(hh), lvl, rho, lmin, lmax, omega);
+    return (int)FEM_OK;
+  });
+}
+
+int fem_apply_smoother_step(fem_handle* hh, int lvl, int mode, double c1, double c2, const double* x,
+                            const double* b, double* d, double* y) {
+  if (!hh || !x || !b || !d || !y) return FEM_ERR_ARG;
+  return guarded(hh, [&]() {
+    Handle& h = *H(hh);
+    ensure_setup(h);
+    const Csr& A = level_matrix_of(h, lvl, 0);
+    const int64_t n = A.n;
+    DevBuf<double> dx(n), db(n), dd(n), dy(n);
+    dx.upload(x, n, h.stream);
+    db.upload(b, n, h.stream);
+    dd.upload(d, n, h.stream);
+    smoother_step_int(h, lvl, mode, c1, c2, dx.get(), db.get(), dd.get(), dy.get());
+    dy.download(y, n, h.stream);
+    dd.download(d, n, h.stream);
     HIP_CHECK(hipStreamSynchronize(h.stream));
     return (int)FEM_OK;
   });
@@ -657,20 +709,28 @@ int fem_read_tetgen(const char* base, int64_t* nv, double** xyz, int64_t* ne, in
   if (!base || !nv || !xyz || !ne || !tets) return FEM_ERR_ARG;
   MeshData m;
   std::string err;
-  if (read_tetgen(base, m, err)) {
+  if (reader_guarded(2, err, [&]() { return read_tetgen(base, m, err); })) {
     g_create_error = err;
     return FEM_ERR_IO;
   }
+  double* px = (double*)std::malloc(sizeof(double) * (m.xyz.size() ? m.xyz.size() : 1));
+  int32_t* pt = (int32_t*)std::malloc(sizeof(int32_t) * (m.conn.size() ? m.conn.size() : 1));
+  int32_t* pl = labels ? (int32_t*)std::malloc(sizeof(int32_t) * (m.labels.size() ? m.labels.size() : 1)) : nullptr;
+  if (!px || !pt || (labels && !pl)) {
+    std::free(px);
+    std::free(pt);
+    std::free(pl);
+    g_create_error = "host out of memory";
+    return FEM_ERR_CAPACITY;
+  }
+  std::copy(m.xyz.begin(), m.xyz.end(), px);
+  std::copy(m.conn.begin(), m.conn.end(), pt);
+  if (labels) std::copy(m.labels.begin(), m.labels.end(), pl);
   *nv = (int64_t)m.xyz.size() / 3;
   *ne = (int64_t)m.conn.size() / 4;
-  *xyz = (double*)std::malloc(sizeof(double) * (m.xyz.size() ? m.xyz.size() : 1));
-  *tets = (int32_t*)std::malloc(sizeof(int32_t) * (m.conn.size() ? m.conn.size() : 1));
-  std::copy(m.xyz.begin(), m.xyz.end(), *xyz);
-  std::copy(m.conn.begin(), m.conn.end(), *tets);
-  if (labels) {
-    *labels = (int32_t*)std::malloc(sizeof(int32_t) * (m.labels.size() ? m.labels.size() : 1));
-    std::copy(m.labels.begin(), m.labels.end(), *labels);
-  }
+  *xyz = px;
+  *tets = pt;
+  if (labels) *labels = pl;
   return FEM_OK;
 }
 
@@ -678,16 +738,24 @@ int fem_read_trimesh(const char* filename, int64_t* nv, double** xyz, int64_t* n
   if (!filename || !nv || !xyz || !ne || !tris) return FEM_ERR_ARG;
   MeshData m;
   std::string err;
-  if (read_trimesh(filename, m, err)) {
+  if (reader_guarded(2, err, [&]() { return read_trimesh(filename, m, err); })) {
     g_create_error = err;
     return FEM_ERR_IO;
   }
+  double* px = (double*)std::malloc(sizeof(double) * (m.xyz.size() ? m.xyz.size() : 1));
+  int32_t* pt = (int32_t*)std::malloc(sizeof(int32_t) * (m.conn.size() ? m.conn.size() : 1));
+  if (!px || !pt) {
+    std::free(px);
+    std::free(pt);
+    g_create_error = "host out of memory";
+    return FEM_ERR_CAPACITY;
+  }
+  std::copy(m.xyz.begin(), m.xyz.end(), px);
+  std::copy(m.conn.begin(), m.conn.end(), pt);
   *nv = (int64_t)m.xyz.size() / 3;
   *ne = (int64_t)m.conn.size() / 3;
-  *xyz = (double*)std::malloc(sizeof(double) * (m.xyz.size() ? m.xyz.size() : 1));
-  *tris = (int32_t*)std::malloc(sizeof(int32_t) * (m.conn.size() ? m.conn.size() : 1));
-  std::copy(m.xyz.begin(), m.xyz.end(), *xyz);
-  std::copy(m.conn.begin(), m.conn.end(), *tris);
+  *xyz = px;
+  *tris = pt;
   return FEM_OK;
 }
 
@@ -697,7 +765,7 @@ int fem_read_matlab_array(const char* filename, double* out, int64_t* n) {
   if (!filename || !n) return -1;
   std::vector<double> v;
   std::string err;
-  int rc = read_matlab_array(filename, v, err);
+  int rc = reader_guarded(-1, err, [&]() { return read_matlab_array(filename, v, err); });
   if (rc) {
     g_create_error = err;
     return rc;
@@ -720,7 +788,7 @@ int fem_read_matlab_sparse_coo(const char* filename, int round_to_float, int64_t
   if (!filename || !nrows || !ncols || !nnz) return FEM_ERR_ARG;
   SparseCoo A;
   std::string err;
-  int rc = read_matlab_sparse(filename, A, err);
+  int rc = reader_guarded(1, err, [&]() { return read_matlab_sparse(filename, A, err); });
   if (rc) {
     g_create_error = err;
     return rc;

@@ -90,8 +90,8 @@ struct Hierarchy {
   std::vector<std::unique_ptr<Level>> levels;  // levels[0].A aliases nothing: its own copy
   int coarse = -1;                             // index of the coarsest level
   int64_t nc = 0;                              // coarsest size
-  DevBuf<double> cinv64;                       // nc x nc dense inverse
-  DevBuf<float> cinv32;
+  DevBuf<double> cinv64;                       // nc x nc dense FP64 inverse (host Cholesky at setup),
+                                               // applied in FP64 in every precision mode
   double grid_complexity = 0, operator_complexity = 0;
   double setup_ms = 0;
   int precision = 0;
@@ -233,7 +233,13 @@ void apply_operator_int(Handle& h, const double* x, double* y, int k);
 const Csr& level_matrix_of(const Handle& h, int lvl, int which);
 // y = M x for that matrix through the one-column solve kernels, in the
 // precision the V-cycle applies it (device FP64 arrays in and out)
+// (which 3: the coarsest level's direct solve, lvl must be the coarsest)
 void apply_level_int(Handle& h, int lvl, int which, const double* x, double* y);
+// the smoother data of a level: rho(D^-1 A) estimate, Chebyshev interval, Jacobi weight
+void level_smoother_info(const Handle& h, int lvl, double* rho, double* lmin, double* lmax, double* omega);
+// one smoothing step on level lvl through the V-cycle's kernel (device FP64 arrays)
+void smoother_step_int(Handle& h, int lvl, int mode, double c1, double c2, const double* x, const double* b,
+                       double* d, double* y);
 
 // Profiling: when enabled, HIP events bracket each launch of a kernel family
 // on the handle's stream; they are read back (without stalling the stream

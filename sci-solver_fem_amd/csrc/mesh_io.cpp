@@ -26,6 +26,11 @@ namespace {
 
 bool skip_line(const std::string& line) { return line.empty() || line[0] == '#'; }
 
+// Initial capacity for a list whose length comes from a file header: the
+// vectors grow as lines are read, so a corrupt or absurd count fails with a
+// "expected N" message instead of a huge allocation up front.
+size_t header_reserve(int64_t n) { return (size_t)std::min<int64_t>(std::max<int64_t>(n, 0), (int64_t)1 << 20); }
+
 }  // namespace
 
 int read_tetgen(const std::string& base, MeshData& m, std::string& err) {
@@ -49,7 +54,8 @@ int read_tetgen(const std::string& base, MeshData& m, std::string& err) {
     return 2;
   }
   m.dim = 3;
-  m.xyz.assign(3 * (size_t)nv, 0.0);
+  m.xyz.clear();
+  m.xyz.reserve(3 * header_reserve(nv));
   int64_t i = 0;
   while (i < nv && std::getline(nf, line)) {
     if (skip_line(line)) continue;
@@ -59,9 +65,9 @@ int read_tetgen(const std::string& base, MeshData& m, std::string& err) {
       err = "Bad Node file, line # " + std::to_string(i);
       return 2;
     }
-    m.xyz[3 * i] = x;
-    m.xyz[3 * i + 1] = y;
-    m.xyz[3 * i + 2] = z;
+    m.xyz.push_back(x);
+    m.xyz.push_back(y);
+    m.xyz.push_back(z);
     ++i;
   }
   if (i != nv) {
@@ -81,8 +87,10 @@ int read_tetgen(const std::string& base, MeshData& m, std::string& err) {
     err = "Bad Ele file";
     return 2;
   }
-  m.conn.assign(4 * (size_t)ne, 0);
-  m.labels.assign((size_t)ne, 0);
+  m.conn.clear();
+  m.labels.clear();
+  m.conn.reserve(4 * header_reserve(ne));
+  m.labels.reserve(header_reserve(ne));
   bool zero_based = false;
   int64_t e = 0;
   while (e < ne && std::getline(ef, line)) {
@@ -95,10 +103,10 @@ int read_tetgen(const std::string& base, MeshData& m, std::string& err) {
       return 2;
     }
     for (int a = 0; a < 4; ++a) {
-      m.conn[4 * e + a] = v[a];
+      m.conn.push_back(v[a]);
       if (v[a] == 0) zero_based = true;
     }
-    m.labels[e] = mat;
+    m.labels.push_back(mat);
     ++e;
   }
   if (e != ne) {
@@ -146,7 +154,7 @@ int read_faces_ascii(FILE* f, int64_t nfaces, int skip_before, MeshData& m, std:
     double dummy;
     for (int q = 0; q < skip_before; ++q) is >> dummy;
     int cnt;
-    if (!(is >> cnt) || cnt < 3) {
+    if (!(is >> cnt) || cnt < 3 || cnt > 65536) {
       err = "bad face line";
       return 2;
     }
@@ -218,7 +226,12 @@ int read_trimesh(const std::string& fname, MeshData& m, std::string& err) {
       err = "only ASCII PLY files with x/y vertex properties are supported";
       return 2;
     }
-    m.xyz.assign(3 * nverts, 0.0);
+    if (nverts < 0 || nfaces < 0) {
+      std::fclose(f);
+      err = "negative element count in PLY header";
+      return 2;
+    }
+    m.xyz.reserve(3 * header_reserve(nverts));
     for (int64_t v = 0; v < nverts; ++v) {
       if (!next_token_line(f, line)) {
         std::fclose(f);
@@ -228,9 +241,9 @@ int read_trimesh(const std::string& fname, MeshData& m, std::string& err) {
       std::istringstream is(line);
       std::vector<float> p(nprops, 0.f);  // TriMesh stores float points
       for (int q = 0; q < nprops; ++q) is >> p[q];
-      m.xyz[3 * v] = p[ix];
-      m.xyz[3 * v + 1] = p[iy];
-      m.xyz[3 * v + 2] = iz >= 0 ? p[iz] : 0.f;
+      m.xyz.push_back(p[ix]);
+      m.xyz.push_back(p[iy]);
+      m.xyz.push_back(iz >= 0 ? p[iz] : 0.f);
     }
     rc = read_faces_ascii(f, nfaces, 0, m, err);
   } else if (line.compare(0, 3, "OFF") == 0) {
@@ -244,7 +257,12 @@ int read_trimesh(const std::string& fname, MeshData& m, std::string& err) {
       err = "bad OFF header";
       return 2;
     }
-    m.xyz.assign(3 * nverts, 0.0);
+    if (nverts < 0 || nfaces < 0) {
+      std::fclose(f);
+      err = "negative count in OFF header";
+      return 2;
+    }
+    m.xyz.reserve(3 * header_reserve(nverts));
     for (int64_t v = 0; v < nverts; ++v) {
       if (!next_token_line(f, line)) {
         std::fclose(f);
@@ -253,9 +271,9 @@ int read_trimesh(const std::string& fname, MeshData& m, std::string& err) {
       }
       float x = 0, y = 0, z = 0;
       std::sscanf(line.c_str(), "%f %f %f", &x, &y, &z);
-      m.xyz[3 * v] = x;
-      m.xyz[3 * v + 1] = y;
-      m.xyz[3 * v + 2] = z;
+      m.xyz.push_back(x);
+      m.xyz.push_back(y);
+      m.xyz.push_back(z);
     }
     rc = read_faces_ascii(f, nfaces, 0, m, err);
   } else {
@@ -297,6 +315,21 @@ struct Reader {
     in.read(reinterpret_cast<char*>(p), n);
     if (!in) ok = false;
   }
+  // bytes left after the read position (0 once a read has failed)
+  int64_t remaining() {
+    if (!ok) return 0;
+    const std::streampos here = in.tellg();
+    in.seekg(0, std::ios::end);
+    const std::streampos end = in.tellg();
+    in.seekg(here);
+    if (!in || here < 0 || end < here) {
+      ok = false;
+      return 0;
+    }
+    return (int64_t)(end - here);
+  }
+  // a data element of n bytes must be non-negative and fit in the file
+  bool fits(int64_t n) { return n >= 0 && n <= remaining(); }
 };
 
 }  // namespace
@@ -358,6 +391,10 @@ int read_matlab_sparse(const std::string& fname, SparseCoo& A, std::string& err)
     return 1;
   }
   bpe = r.get<int32_t>();
+  if (!r.fits(bpe)) {
+    err = "truncated sparse matrix file";
+    return 1;
+  }
   std::vector<int32_t> ir(bpe / 4);
   r.bytes(ir.data(), (int64_t)ir.size() * 4);
   r.skip(bpe % 8);
@@ -368,6 +405,10 @@ int read_matlab_sparse(const std::string& fname, SparseCoo& A, std::string& err)
     return 1;
   }
   bpe = r.get<int32_t>();
+  if (!r.fits(bpe)) {
+    err = "truncated sparse matrix file";
+    return 1;
+  }
   std::vector<int32_t> jc(bpe / 4);
   r.bytes(jc.data(), (int64_t)jc.size() * 4);
   r.skip(bpe % 8);
@@ -378,9 +419,13 @@ int read_matlab_sparse(const std::string& fname, SparseCoo& A, std::string& err)
     return 1;
   }
   bpe = r.get<int32_t>();
+  if (!r.fits(bpe)) {
+    err = "truncated sparse matrix file";
+    return 1;
+  }
   std::vector<double> pr(bpe / 8);
   r.bytes(pr.data(), (int64_t)pr.size() * 8);
-  if (!r.ok || (int64_t)jc.size() < (int64_t)ydim + 1) {
+  if (!r.ok || xdim < 0 || ydim < 0 || (int64_t)jc.size() < (int64_t)ydim + 1) {
     err = "truncated sparse matrix file";
     return 1;
   }
@@ -454,6 +499,10 @@ int read_matlab_array(const std::string& fname, std::vector<double>& out, std::s
     return -1;
   }
   uint32_t len = r.get<uint32_t>();
+  if (!r.fits((int64_t)len)) {
+    err = "truncated array file";
+    return -1;
+  }
   out.assign(len / 8, 0.0);
   r.bytes(out.data(), (int64_t)out.size() * 8);
   if (!r.ok) {

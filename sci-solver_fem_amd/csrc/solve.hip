@@ -74,7 +74,6 @@ struct LV<double> {
   static double* b(Level& L) { return L.b64.get(); }
   static double* r(Level& L) { return L.r64.get(); }
   static double* d(Level& L) { return L.d64.get(); }
-  static const double* cinv(const Hierarchy& H) { return H.cinv64.get(); }
 };
 template <>
 struct LV<float> {
@@ -83,7 +82,6 @@ struct LV<float> {
   static float* b(Level& L) { return L.b32.get(); }
   static float* r(Level& L) { return L.r32.get(); }
   static float* d(Level& L) { return L.d32.get(); }
-  static const float* cinv(const Hierarchy& H) { return H.cinv32.get(); }
 };
 
 template <typename V, typename T, int MODE, int G>
@@ -415,15 +413,17 @@ __global__ void convert_kernel(int64_t m, const Tin* __restrict__ a, Tout* __res
   if (t < m) b[t] = (Tout)a[t];
 }
 
-// dense coarse solve x = C b, C row-major nc x nc; one wave per row
+// dense coarse solve x = C b with the FP64 inverse C (row-major nc x nc) in
+// every precision: b is widened to FP64, the products are summed in FP64 and
+// only the result is rounded to the level's vector type; one wave per row
 template <typename T>
-__global__ void __launch_bounds__(64) coarse_kernel(int64_t nc, int k, const T* __restrict__ C, const T* __restrict__ b,
-                                                   T* __restrict__ x) {
+__global__ void __launch_bounds__(64) coarse_kernel(int64_t nc, int k, const double* __restrict__ C,
+                                                   const T* __restrict__ b, T* __restrict__ x) {
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   for (int c = 0; c < k; ++c) {
     double acc = 0.0;
-    for (int64_t j = lane; j < nc; j += 64) acc += (double)C[i * nc + j] * (double)b[j * k + c];
+    for (int64_t j = lane; j < nc; j += 64) acc += C[i * nc + j] * (double)b[j * k + c];
     acc = wave_sum(acc);
     if (lane == 0) x[i * k + c] = (T)acc;
   }
@@ -861,6 +861,12 @@ __global__ void pcg_p_kernel(int64_t m, int k, const Z* __restrict__ z, double* 
   p[t] = (double)z[t] + scal[S_BETA * k + c] * p[t];
 }
 
+// x[:, c] = 0 for every column c whose bit is set in mask (k <= 64 columns)
+__global__ void zero_cols_kernel(int64_t m, int k, unsigned long long mask, double* __restrict__ x) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m && ((mask >> (t % k)) & 1ull)) x[t] = 0.0;
+}
+
 __global__ void copy_kernel(int64_t m, const double* __restrict__ a, double* __restrict__ b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) b[t] = a[t];
@@ -1132,7 +1138,7 @@ T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr, bool firs
   hipStream_t s = h.stream;
   const Csr& A = level_A(h, l);
   if (l == H.coarse) {
-    coarse_kernel<T><<<(unsigned)H.nc, 64, 0, s>>>(H.nc, k, LV<T>::cinv(H), b, LV<T>::x(L));
+    coarse_kernel<T><<<(unsigned)H.nc, 64, 0, s>>>(H.nc, k, H.cinv64.get(), b, LV<T>::x(L));
     HIP_LAUNCH_CHECK();
     return LV<T>::x(L);
   }
@@ -1423,15 +1429,84 @@ void apply_operator_int(Handle& h, const double* x, double* y, int k) {
 const Csr& level_matrix_of(const Handle& h, int lvl, int which) {
   const Hierarchy& H = h.H;
   if (lvl < 0 || lvl >= (int)H.levels.size()) fail(FEM_ERR_ARG, "level out of range");
-  if (which == 0) return level_A(h, lvl);
+  if (which == 0 || which == 3) return level_A(h, lvl);
   if (which != 1 && which != 2) fail(FEM_ERR_ARG, "which: 0 operator, 1 prolongator, 2 restriction");
   if (lvl == H.coarse) fail(FEM_ERR_ARG, "the coarsest level has no prolongator or restriction");
   return which == 1 ? H.levels[lvl]->P : H.levels[lvl]->R;
 }
 
-void apply_level_int(Handle& h, int lvl, int which, const double* x, double* y) {
-  const Csr& M = level_matrix_of(h, lvl, which);
+void level_smoother_info(const Handle& h, int lvl, double* rho, double* lmin, double* lmax, double* omega) {
+  const Hierarchy& H = h.H;
+  if (lvl < 0 || lvl >= (int)H.levels.size()) fail(FEM_ERR_ARG, "level out of range");
+  if (lvl == H.coarse) fail(FEM_ERR_ARG, "the coarsest level is solved directly, it has no smoother");
+  const Level& L = *H.levels[lvl];
+  const Smoother S = smoother_for(h, L);
+  if (rho) *rho = L.rho;
+  if (lmin) *lmin = S.lmin;
+  if (lmax) *lmax = S.lmax;
+  if (omega) *omega = S.omega;
+}
+
+namespace {
+template <typename T>
+void smoother_step_t(Handle& h, const Csr& A, const T* dinv, int mode, double c1, double c2, const T* x, const T* b,
+                     T* d, T* y) {
+  if (mode == 0)
+    spmv_op<T, kJacobi>(A, 1, x, nullptr, b, dinv, nullptr, y, (T)c1, T(0), h.stream);
+  else
+    spmv_op<T, kCheb>(A, 1, x, nullptr, b, dinv, d, y, (T)c1, (T)c2, h.stream);
+}
+}  // namespace
+
+void smoother_step_int(Handle& h, int lvl, int mode, double c1, double c2, const double* x, const double* b,
+                       double* d, double* y) {
+  const Hierarchy& H = h.H;
+  if (lvl < 0 || lvl >= (int)H.levels.size() || lvl == H.coarse) fail(FEM_ERR_ARG, "level has no smoother");
+  if (mode != 0 && mode != 1) fail(FEM_ERR_ARG, "mode: 0 Jacobi, 1 Chebyshev");
+  const Csr& A = level_A(h, lvl);
+  const Level& L = *H.levels[lvl];
   hipStream_t s = h.stream;
+  const int64_t n = A.n;
+  if (H.precision == 0) {
+    smoother_step_t<double>(h, A, L.dinv64.get(), mode, c1, c2, x, b, d, y);
+  } else {
+    if (!A.v32.size() || (A.nslices > 0 && !A.s32.size()) || !L.dinv32.size())
+      fail(FEM_ERR_STATE, "no FP32 copy of this level");
+    DevBuf<float> xf(n), bf(n), df(n), yf(n);
+    convert_kernel<double, float><<<grid_for(n, 256), 256, 0, s>>>(n, x, xf.get());
+    convert_kernel<double, float><<<grid_for(n, 256), 256, 0, s>>>(n, b, bf.get());
+    convert_kernel<double, float><<<grid_for(n, 256), 256, 0, s>>>(n, d, df.get());
+    HIP_LAUNCH_CHECK();
+    smoother_step_t<float>(h, A, L.dinv32.get(), mode, c1, c2, xf.get(), bf.get(), df.get(), yf.get());
+    convert_kernel<float, double><<<grid_for(n, 256), 256, 0, s>>>(n, yf.get(), y);
+    convert_kernel<float, double><<<grid_for(n, 256), 256, 0, s>>>(n, df.get(), d);
+    HIP_LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+void apply_level_int(Handle& h, int lvl, int which, const double* x, double* y) {
+  hipStream_t s = h.stream;
+  if (which == 3) {
+    // the coarsest-level solve the V-cycle applies (FP64 inverse in every
+    // precision; b and x in the level's vector type)
+    const Hierarchy& H = h.H;
+    if (lvl != H.coarse) fail(FEM_ERR_ARG, "which 3 (coarse solve) needs the coarsest level");
+    const int64_t nc = H.nc;
+    if (H.precision == 0) {
+      coarse_kernel<double><<<(unsigned)nc, 64, 0, s>>>(nc, 1, H.cinv64.get(), x, y);
+      HIP_LAUNCH_CHECK();
+    } else {
+      DevBuf<float> bf(nc), xf(nc);
+      convert_kernel<double, float><<<grid_for(nc, 256), 256, 0, s>>>(nc, x, bf.get());
+      coarse_kernel<float><<<(unsigned)nc, 64, 0, s>>>(nc, 1, H.cinv64.get(), bf.get(), xf.get());
+      convert_kernel<float, double><<<grid_for(nc, 256), 256, 0, s>>>(nc, xf.get(), y);
+      HIP_LAUNCH_CHECK();
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return;
+  }
+  const Csr& M = level_matrix_of(h, lvl, which);
   if (h.H.precision == 0) {
     spmv_op<double, kAssign>(M, 1, x, y, nullptr, (const double*)nullptr, nullptr, nullptr, 0.0, 0.0, s);
   } else {
@@ -1730,6 +1805,15 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     for (int c = 0; c < k; ++c)
       if (rel[c] > tol) status = FEM_ERR_NOT_CONVERGED;
   } else {
+    // a zero right-hand side has the exact solution x = 0, whatever the
+    // initial guess (its column then starts, and stays, converged)
+    unsigned long long zmask = 0;
+    for (int c = 0; c < k; ++c)
+      if (!(bb[c] > 0)) zmask |= 1ull << c;
+    if (zmask) {
+      zero_cols_kernel<<<grid_for(m, 256), 256, 0, s>>>(m, k, zmask, h.kx.get());
+      HIP_LAUNCH_CHECK();
+    }
     // r = b - A x0
     spmv_op<double, kResid>(h.A, k, h.kx.get(), h.kr.get(), h.kb.get(), (const double*)nullptr, nullptr, nullptr,
                             0.0, 0.0, s);

@@ -32,8 +32,13 @@ int main(int argc, char** argv) {
     cfg.solveFEM(&x_h, &b_h);
     if (cfg.writeMatlabArray("output.mat", x_h)) std::cerr << "failed to write matlab file." << std::endl;
     std::vector<double> vals(x_h.begin(), x_h.end());
-    size_t pos = cfg.filename_.find_last_of("/");
-    std::string outname = (pos == std::string::npos) ? cfg.filename_ : cfg.filename_.substr(pos + 1);
+    // output base name: the mesh file name without its folders and extension
+    // (example2 writes <name>.vtk next to the working directory)
+    std::string outname = cfg.filename_;
+    const size_t slash = outname.find_last_of("/\\");
+    if (slash != std::string::npos) outname = outname.substr(slash + 1);
+    const size_t dot = outname.find_last_of('.');
+    if (dot != std::string::npos) outname = outname.substr(0, dot);
     cfg.writeVTK(vals, outname);
   } catch (const std::exception& e) {
     std::cerr << e.what() << std::endl;

@@ -113,3 +113,72 @@ def test_torch_autograd_function(fem, gpu, oracle, meshgen):
     assert np.linalg.norm(gk - g_ref) / np.linalg.norm(g_ref) < 1e-7
     assert np.linalg.norm(gb - lam) / np.linalg.norm(lam) < 1e-7
     s.close()
+
+
+def test_torch_autograd_two_applies_different_kappa(fem, gpu, oracle, meshgen):
+    """Two forward applies on one solver with different kappa and one backward
+    through both: each node's adjoint must use its own K (the solver holds the
+    second kappa when backward starts)."""
+    import torch
+    report("[gpu] torch.autograd: two applies, different kappa, one backward")
+    xyz, tets, kappa, kappa2, b = _setup(meshgen, N=7, seed=13)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    _tight(s, precision=0)
+    s.set_params(tolerance=1e-11)
+    dev = torch.device("cuda:0")
+    k1 = torch.tensor(kappa, dtype=torch.float64, device=dev, requires_grad=True)
+    k2 = torch.tensor(kappa2, dtype=torch.float64, device=dev, requires_grad=True)
+    b_t = torch.tensor(b, dtype=torch.float64, device=dev)
+    w1 = torch.tensor(meshgen.noise(21, xyz.shape[0], 1.0), dtype=torch.float64, device=dev)
+    w2 = torch.tensor(meshgen.noise(22, xyz.shape[0], 1.0), dtype=torch.float64, device=dev)
+    Fn = fem.FEMSolveFunction
+    u1 = Fn.apply(k1, b_t, s)
+    u2 = Fn.apply(k2, b_t, s)
+    L = (w1 * u1).sum() + 0.5 * ((u2 - w2) ** 2).sum()
+    L.backward()
+    for kap, kt, dLdu_fn in ((kappa, k1, lambda u: w1.cpu().numpy()),
+                             (kappa2, k2, lambda u: u - w2.cpu().numpy())):
+        lu = spla.splu(oracle.scipy_matrix(xyz, tets, kap, 1e-2).tocsc())
+        u_ref = lu.solve(b)
+        lam = lu.solve(dLdu_fn(u_ref))
+        g_ref = oracle.element_grad(xyz, tets, lam, u_ref)
+        g = kt.grad.cpu().numpy()
+        assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-7
+    s.close()
+
+
+def test_torch_inputs_from_async_kernels_on_default_stream(fem, gpu, oracle, meshgen):
+    """b and the initial guess produced by asynchronous torch kernels on the
+    default stream immediately before the call: the library must see their
+    final values (it runs on a stream of its own)."""
+    import torch
+    report("[gpu] torch default-stream producers before solve / apply_operator")
+    xyz, tets, kappa, _, b = _setup(meshgen, N=9, seed=17)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    _tight(s, precision=0)
+    A = oracle.scipy_matrix(xyz, tets, kappa, 1e-2)
+    dev = torch.device("cuda:0")
+    assert torch.cuda.current_stream().cuda_stream == 0
+    n = xyz.shape[0]
+    big = torch.randn(4096, 4096, dtype=torch.float64, device=dev)
+    for trial in range(3):
+        torch.cuda.synchronize()
+        # a long queue of work ahead of the producer, then b = 0 + b_host * (trial + 1)
+        for _ in range(4):
+            big = big @ big
+            big = big / big.norm()
+        b_t = torch.zeros(n, dtype=torch.float64, device=dev) + big[0, 0] * 0
+        b_t += torch.tensor(b, dtype=torch.float64, device=dev) * (trial + 1)
+        x, info = s.solve(b_t)
+        xd = spla.spsolve(A.tocsc(), b * (trial + 1))
+        assert info["status"] == 0
+        assert np.linalg.norm(x.cpu().numpy() - xd) / np.linalg.norm(xd) < 1e-8
+        for _ in range(4):
+            big = big @ big
+            big = big / big.norm()
+        xin = torch.zeros(n, dtype=torch.float64, device=dev) + big[0, 0] * 0
+        xin += torch.tensor(b, dtype=torch.float64, device=dev)
+        y = s.apply_operator(xin)
+        yd = A @ b
+        assert np.abs(y.cpu().numpy() - yd).max() <= 1e-13 * np.abs(yd).max()
+    s.close()

@@ -154,3 +154,109 @@ def test_bench_algorithmic_bytes_of_assembly():
     n, ne, nnz = 161 ** 3, 6 * 160 ** 3, 61_979_041
     assert ne == 24_576_000
     assert round(bench.assembly_bytes(n, ne, nnz) / 1e6, 1) == 1185.8
+
+
+def test_corrupt_sizes_return_codes_not_aborts(fem, tmp_path):
+    """Header counts and element sizes taken from a file never drive an
+    allocation by themselves: an absurd or negative count is a status code
+    (the reference's 1 / -1 for MAT files), not an abort through the C ABI."""
+    # .node announcing 2e9 vertices but holding two
+    (tmp_path / "huge.node").write_text("2000000000 3 0 0\n1 0 0 0\n2 1 0 0\n")
+    (tmp_path / "huge.ele").write_text("1 4 0\n1 1 2 1 2\n")
+    with pytest.raises(fem.FemError):
+        fem.read_tetgen(str(tmp_path / "huge"))
+    # .ele announcing more elements than it holds
+    (tmp_path / "short.node").write_text("4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n")
+    (tmp_path / "short.ele").write_text("900000000 4 0\n1 1 2 3 4\n")
+    with pytest.raises(fem.FemError):
+        fem.read_tetgen(str(tmp_path / "short"))
+    # PLY / OFF with absurd counts
+    (tmp_path / "big.off").write_text("OFF\n-5 1 0\n")
+    with pytest.raises(fem.FemError):
+        fem.read_trimesh(str(tmp_path / "big.off"))
+    (tmp_path / "big2.off").write_text("OFF\n3000000000 1 0\n0 0 0\n")
+    with pytest.raises(fem.FemError):
+        fem.read_trimesh(str(tmp_path / "big2.off"))
+    (tmp_path / "poly.off").write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n0 1 0\n2000000000 0 1 2\n")
+    with pytest.raises(fem.FemError):
+        fem.read_trimesh(str(tmp_path / "poly.off"))
+
+    # MAT v5 sparse: a valid header whose row-index element claims 2 GB
+    good = (REFDATA / "simple.mat").read_bytes()
+    (_, _), r, c, v = fem.read_matlab_sparse_coo(str(REFDATA / "simple.mat"))
+    ir_at = _mat_sparse_ir_offset(good)
+    for size in (0x7FFFFFF0, -8):
+        bad = bytearray(good)
+        struct.pack_into("<i", bad, ir_at + 4, size)
+        p = tmp_path / f"bad_ir_{size & 0xffff}.mat"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(fem.FemError) as e:
+            fem.read_matlab_sparse_coo(str(p))
+        assert e.value.status == 1
+    # truncated copies of a valid sparse and a valid dense file
+    for cut in (140, 200, len(good) // 2, len(good) - 3):
+        p = tmp_path / f"trunc_{cut}.mat"
+        p.write_bytes(good[:cut])
+        with pytest.raises(fem.FemError) as e:
+            fem.read_matlab_sparse_coo(str(p))
+        assert e.value.status == 1
+    dense = (REFDATA / "simpleb.mat").read_bytes()
+    for cut in (150, len(dense) - 5):
+        p = tmp_path / f"trunc_dense_{cut}.mat"
+        p.write_bytes(dense[:cut])
+        with pytest.raises(fem.FemError) as e:
+            fem.read_matlab_array(str(p))
+        assert e.value.status == -1
+    # dense data element claiming 4 GB
+    bad = bytearray(dense)
+    struct.pack_into("<I", bad, len(dense) - 8 * 5000 - 4, 0xFFFFFFF8)
+    p = tmp_path / "dense_huge.mat"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(fem.FemError) as e:
+        fem.read_matlab_array(str(p))
+    assert e.value.status == -1
+
+
+def _mat_sparse_ir_offset(raw: bytes) -> int:
+    """Byte offset of the row-index data element tag in a MAT v5 sparse file
+    (after the 128-byte header, the matrix tag, array flags, dimensions and
+    the name element)."""
+    off = 128 + 8          # matrix tag
+    off += 16              # array flags element
+    off += 16              # dimensions element (two int32)
+    t, n = struct.unpack_from("<HH", raw, off)
+    if n:                  # small data element: 4-byte tag + data padded to 4
+        off += 4 + ((n + 3) // 4) * 4
+    else:
+        t, n = struct.unpack_from("<ii", raw, off)
+        off += 8 + ((n + 7) // 8) * 8
+    return off
+
+
+def test_read_off_triangles_and_polygons(fem, tmp_path):
+    """TriMesh's OFF reader (TriMesh_io.cu:692-708): counts on the header or
+    the next line, float vertices, polygon faces split into a fan."""
+    p = tmp_path / "quad.off"
+    p.write_text("OFF\n# comment line\n5 2 0\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n0.5 0.5 0.25\n"
+                 "4 0 1 2 3\n3 0 1 4\n")
+    xyz, tris = fem.read_trimesh(str(p))
+    assert xyz.shape == (5, 3) and xyz[4, 2] == 0.25
+    assert tris.tolist() == [[0, 1, 2], [0, 2, 3], [0, 1, 4]]
+    p2 = tmp_path / "inline.off"
+    p2.write_text("OFF 3 1 0\n0 0 0\n1 0 0\n0 1 0\n3 0 1 2\n")
+    xyz, tris = fem.read_trimesh(str(p2))
+    assert xyz.shape == (3, 3) and tris.tolist() == [[0, 1, 2]]
+    # the same triangle mesh through PLY and OFF reads identically
+    xs, ts = fem.read_trimesh(str(REFDATA / "sphere_290verts.ply"))
+    lines = ["OFF", f"{xs.shape[0]} {ts.shape[0]} 0"]
+    lines += [" ".join(f"{c:.9g}" for c in row) for row in xs]
+    lines += ["3 " + " ".join(str(int(i)) for i in t) for t in ts]
+    p3 = tmp_path / "sphere.off"
+    p3.write_text("\n".join(lines) + "\n")
+    xo, to = fem.read_trimesh(str(p3))
+    assert np.array_equal(to, ts) and np.array_equal(xo, xs)
+    # out-of-range face index
+    p4 = tmp_path / "badidx.off"
+    p4.write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n0 1 0\n3 0 1 7\n")
+    with pytest.raises(fem.FemError):
+        fem.read_trimesh(str(p4))
