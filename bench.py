@@ -162,6 +162,209 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib, 
     return res
 
 
+def _median_solve(torch, s, b, reps=3):
+    """One warm-up solve, then the median wall time of `reps` solves."""
+    x, info = s.solve(b, raise_on_nonconvergence=True)
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, info = s.solve(b, raise_on_nonconvergence=True)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), x, info
+
+
+def _gpu_row(torch, fem, P, precision, b, ref=None, label=None, params=None):
+    """Assembly, setup and solve of one workload on the GPU (FP64 or mixed)."""
+    dev = torch.device("cuda:0")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    s = fem.FEMSolver.from_arrays(P["dim"], P["xyz"], P["conn"], kappa=P["kappa"], mass_coeff=P["mass"])
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t
+    s.set_params(**(params or solver_params(precision)))
+    t = time.perf_counter()
+    s.setup()
+    t_setup = time.perf_counter() - t
+    bt = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
+    dt, x, info = _median_solve(torch, s, bt)
+    xh = x.cpu().numpy()
+    k = 1 if b.ndim == 1 else b.shape[1]
+    n = P["xyz"].shape[0]
+    row = dict(path=label or ("GPU FP64" if precision == 0 else "GPU FP32-hier / FP64 outer"), assembly_s=t_asm,
+               setup_s=t_setup, solve_s=dt, iterations=info["iterations"], rel_residual=info["rel_residual"],
+               dof_per_s=n * k / dt, levels=info["num_levels"], operator_complexity=info["operator_complexity"])
+    if ref is not None:
+        row["rel_err"] = float(np.linalg.norm(xh - ref) / np.linalg.norm(ref))
+    return s, xh, row
+
+
+def _cpu_row(orc, P, b):
+    """The oracle's restatement of the reference algorithm on the host: the
+    reference defaults first (5/5 partition-local Jacobi sweeps of weight 1,
+    512-row partitions); where that stalls, weight 0.5 with one partition per
+    level (noted)."""
+    t = time.perf_counter()
+    rp, ci, v = orc.assemble(P["xyz"], P["conn"], P["kappa"], P["mass"])
+    t_asm = time.perf_counter() - t
+    n = P["xyz"].shape[0]
+    note = "reference defaults"
+    # maxIters 100: the reference's own default (FEMSolver.cu:11-33)
+    res = orc.amg_pcg(None, None, None, None, b, tol=1e-8, max_iters=100, A=(rp, ci, v))
+    if not res["converged"]:
+        note = f"reference defaults stall ({res['iterations']} it, residual {res['rel_residual']:.1e}); " \
+               f"weight 0.5, one partition per level"
+        res = orc.amg_pcg(None, None, None, None, b, tol=1e-8, max_iters=1000, A=(rp, ci, v), smoother_weight=0.5,
+                          part_max=n)
+    return res, dict(path="CPU reference algorithm (oracle)", cores=res["threads"], assembly_s=t_asm,
+                     setup_s=res["setup_s"], solve_s=res["solve_s"], iterations=res["iterations"],
+                     rel_residual=res["rel_residual"], dof_per_s=n / res["solve_s"], notes=note,
+                     levels=res["levels"], operator_complexity=res["operator_complexity"])
+
+
+def _fp64_reference(orc, P, b):
+    """FP64 reference solution: SuperLU for the 2D workload; the oracle's
+    FP64-outer PCG to a true relative residual of 1e-13 for the 3D ones (a
+    direct factorisation of the 49^3 system alone takes minutes on the host)."""
+    import scipy.sparse.linalg as spla
+    rp, ci, v = orc.assemble(P["xyz"], P["conn"], P["kappa"], P["mass"])
+    n = P["xyz"].shape[0]
+    if P["dim"] == 2:
+        import scipy.sparse as sp
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        return spla.spsolve(A.tocsc(), b), "SuperLU"
+    res = orc.amg_pcg(None, None, None, None, b, tol=1e-13, max_iters=3000, A=(rp, ci, v), smoother_weight=0.5,
+                      part_max=n)
+    return res["x"], f"CPU PCG to true residual {res['rel_residual']:.1e}"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_configs(a):
+    """Every workload of the benchmark definition (C1-C5) once: GPU FP64 and
+    mixed rows, the CPU reference-algorithm rows (C1-C3), the adjoint (C4,
+    C5) and the error against an FP64 reference solution; JSON to --out."""
+    import torch
+    fem = load_package()
+    mg = fem.meshgen
+    sys.path.insert(0, str(ROOT / "tests"))
+    from conftest import load_oracle
+    orc = load_oracle()
+    rows = []
+    meta = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "gpu": torch.cuda.get_device_name(0)}
+    log(f"[configs] host {meta['cpu_model']} ({meta['nproc']} cpus, OMP_NUM_THREADS={meta['omp_num_threads']})")
+    warm = fem.FEMSolver.from_arrays(3, *mg.cube_mesh(9, seed=1), mass_coeff=1e-2)
+    for prec in (1, 0):
+        warm.set_params(**dict(solver_params(prec), topSize=16))
+        warm.solve(np.ones(9 ** 3))
+    warm.close()
+
+    def add(cfg, r):
+        r["cfg"] = cfg
+        rows.append(r)
+        log(f"[configs] {cfg} {r['path']}: " + ", ".join(
+            f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}" for k, v in r.items() if k not in ("cfg", "path")))
+
+    for cfg in ("c1_square65_fp64", "c2_cube49_fp64", "c3_cube97_mixed"):
+        P = mg.make_problem(**mg.CONFIGS[cfg])
+        b = P["b"] if P["b"].ndim == 1 else np.ascontiguousarray(P["b"][:, 0])
+        ref, how = _fp64_reference(orc, P, b)
+        log(f"[configs] {cfg}: FP64 reference by {how}")
+        res, r = _cpu_row(orc, P, b)
+        r["rel_err"] = float(np.linalg.norm(res["x"] - ref) / np.linalg.norm(ref))
+        r["reference"] = how
+        add(cfg, r)
+        precs = (0,) if cfg != "c3_cube97_mixed" else (0, 1)
+        for prec in precs:
+            params = dict(solver_params(prec), topSize=64) if P["dim"] == 2 else None
+            s, x, r = _gpu_row(torch, fem, P, prec, b, ref, params=params)
+            r["reference"] = how
+            add(cfg, r)
+            s.close()
+
+    # C4: adjoint on the 49^3 mesh
+    cfg = "c4_cube49_adjoint"
+    P = mg.make_problem(**mg.CONFIGS[cfg])
+    tight = dict(solver_params(0), tolerance=1e-12, maxIters=1000)
+    s2 = fem.FEMSolver.from_arrays(3, P["xyz"], P["conn"], kappa=mg.lognormal_kappa(101, P["conn"].shape[0], 1.0),
+                                   mass_coeff=P["mass"])
+    s2.set_params(**tight)
+    u2, _ = s2.solve(P["b"])
+    s2.close()
+    n = P["xyz"].shape[0]
+    u_obs = u2 + mg.noise(7, n, 1e-3)
+    s, u, r = _gpu_row(torch, fem, P, 0, P["b"], params=tight, label="GPU adjoint (FP64 outer), forward solve")
+    dev = torch.device("cuda:0")
+    ut = torch.from_numpy(u).to(dev)
+    dLdu = torch.from_numpy(u - u_obs).to(dev)
+    s.adjoint_grad(ut, dLdu)
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g, lam, ainfo = s.adjoint_grad(ut, dLdu)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    t_adj = float(np.median(ts))
+    g = g.cpu().numpy()
+    fx = np.load(ROOT / "tests" / "golden" / "c4_gradient.npz")
+    r.update(adjoint_s=t_adj, adjoint_over_forward=t_adj / r["solve_s"], adjoint_iterations=ainfo["iterations"],
+             grad_rel_err_vs_fp64_cpu=float(np.linalg.norm(g[fx["idx"]] - fx["g"]) / np.linalg.norm(fx["g"])),
+             grad_norm_rel_err=float(abs(np.linalg.norm(g) - float(fx["g_norm"])) / float(fx["g_norm"])))
+    add(cfg, r)
+    s.close()
+
+    # C5: FP64 and mixed (1 RHS), mixed 16 RHS, adjoint (1 RHS); error against the
+    # GPU FP64 path at 1e-12 (no host reference at 4.17M DOF within the run)
+    cfg = "c5_cube161_mixed16"
+    P = mg.make_problem(**mg.CONFIGS[cfg])
+    B = P["b"]
+    b = np.ascontiguousarray(B[:, 0])
+    s, ref, _ = _gpu_row(torch, fem, P, 0, b, params=dict(solver_params(0), tolerance=1e-12, maxIters=1000))
+    s.close()
+    how = "GPU FP64 path at 1e-12"
+    s, x64, r = _gpu_row(torch, fem, P, 0, b, ref)
+    r["reference"] = how
+    add(cfg, r)
+    s.close()
+    s, xm, r = _gpu_row(torch, fem, P, 1, b, ref)
+    r["reference"] = how
+    add(cfg, r)
+    Bt = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
+    dtb, X, ib = _median_solve(torch, s, Bt, reps=2)
+    add(cfg, dict(path="GPU FP32-hier, 16 RHS", solve_s=dtb, iterations=ib["iterations"],
+                  rel_residual=ib["rel_residual"], dof_rhs_per_s=P["xyz"].shape[0] * 16 / dtb))
+    del Bt, X
+    ut = torch.from_numpy(xm).to(dev)
+    dLdu = torch.from_numpy(mg.noise(5, P["xyz"].shape[0], 1e-3)).to(dev)
+    s.adjoint_grad(ut, dLdu)
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, ainfo = s.adjoint_grad(ut, dLdu)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    t_adj = float(np.median(ts))
+    fwd = [q for q in rows if q["cfg"] == cfg and q["path"].startswith("GPU FP32-hier / FP64")][0]["solve_s"]
+    add(cfg, dict(path="GPU adjoint, 1 RHS (FP32-hier)", adjoint_s=t_adj, adjoint_over_forward=t_adj / fwd,
+                  iterations=ainfo["iterations"]))
+    s.close()
+    out = {"meta": meta, "rows": rows, "command": " ".join(["python", "bench.py"] + sys.argv[1:])}
+    Path(a.out).write_text(json.dumps(out, indent=1) + "\n")
+    log(f"[configs] wrote {a.out}")
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         # bench.py --pmc-summary FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_MIB ALGO_BYTES "command"
@@ -184,12 +387,18 @@ def main():
     ap.add_argument("--skip", default="", help="comma list of: fp64,fp16,batched,adjoint,cpu")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (calibrates the HBM byte counters) and exit")
+    ap.add_argument("--configs", action="store_true",
+                    help="run every workload (C1-C5) once with per-config rows instead of the headline line")
+    ap.add_argument("--out", default="gpurun_out/configs.json", help="--configs: JSON output path")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
               file=sys.stderr)
         sys.exit(2)
     skip = set(filter(None, a.skip.split(",")))
+    if a.configs:
+        run_configs(a)
+        return
 
     import torch
     if a.copy_probe:

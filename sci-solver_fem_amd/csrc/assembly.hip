@@ -9,11 +9,12 @@
 //               src/core/cuda/perform_element_loop_2D.cuh:288-372):
 //              K_e = co * |T| grad(phi_a).grad(phi_b) + M_e, scattered with FP64 CAS atomics.
 //
-// Here the scatter is replaced by a gather: one thread owns one matrix row,
-// walks the elements incident to its vertex in ascending element order and
-// sums their contributions into the row in LDS.  No float atomics, a fixed
-// summation order (bit-reproducible) and A is bitwise symmetric because both
-// rows of a pair see the same element contributions in the same order.
+// Here the scatter is replaced by two passes: one thread per element computes
+// the element's entries once (coalesced int4 connectivity and coefficient
+// reads), then one thread per matrix row gathers the rows of its incident
+// element matrices in ascending element order.  No float atomics, a fixed
+// summation order (bit-reproducible), and A is bitwise symmetric because both
+// entries of a pair add the same stored element values in the same order.
 #include "fem_internal.hpp"
 
 #include <algorithm>
@@ -25,7 +26,6 @@ namespace {
 
 constexpr int kRowBlock = 64;   // threads (rows) per block for row-owner kernels
 constexpr int kRowCap = 64;     // LDS row capacity; longer rows use the global path
-constexpr int kAsmCap = 32;     // LDS row capacity of the value assembly (12 B per slot)
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
   v &= 0x1fffffull;
@@ -262,7 +262,158 @@ __device__ __forceinline__ void element_geometry(const double4* __restrict__ X, 
   }
 }
 
-__device__ __forceinline__ int find_col(const int* cols, int len, int c) {
+// ---- element-centric assembly ---------------------------------------------------
+// Pass 1 (element_values_kernel): one thread per element computes, once, the
+// element's off-diagonal stiffness entries S_ab = kappa |T| grad(phi_a).grad(phi_b)
+// (a < b: 01 02 03 12 13 23 for tets, 01 02 12 for triangles) and its mass value
+// m = mass_coeff |T| / 20 (tets) or / 12 (triangles; signed |T| with flags bit 0),
+// stored as one record per element (8 doubles for tets, 4 for triangles).
+// The element matrix is then K_ab = S_ab + m (a != b) and, because the
+// stiffness rows of a P1 element sum to zero, K_aa = -sum_{b != a} S_ab + 2 m.
+//
+// Pass 2 (assemble_rows_*): one thread per matrix row walks the elements
+// incident to its vertex in ascending element order and adds the row of each
+// element matrix into the row's entries.  Every entry is a sum over its
+// elements in ascending element order, and entry (i, j) and entry (j, i) add
+// the same stored S_ab + m, so A is deterministic and bitwise symmetric
+// without float atomics.  Rows of up to W entries keep their columns and
+// sums in registers (a contribution is added to the slot whose column
+// matches: compare-and-select over the W slots, no LDS and no search);
+// longer rows sum in place in the CSR value array.
+
+template <int NPE>
+struct EvLayout;
+template <>
+struct EvLayout<4> {
+  static constexpr int kStride = 8;  // 6 S_ab, m, unused
+  static constexpr int kM = 6;
+  // slot of S_ab for corners a != b
+  __device__ static int slot(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo == 0 ? hi - 1 : (lo == 1 ? hi + 1 : 5);
+  }
+};
+template <>
+struct EvLayout<3> {
+  static constexpr int kStride = 4;  // 3 S_ab, m
+  static constexpr int kM = 3;
+  __device__ static int slot(int a, int b) { return a + b - 1; }
+};
+
+template <int NPE>
+__global__ void __launch_bounds__(256) element_values_kernel(int64_t ne, int signed_mass, double mass_coeff,
+                                                             const int4* __restrict__ conn,
+                                                             const double4* __restrict__ X,
+                                                             const double* __restrict__ kappa,
+                                                             double* __restrict__ ev) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int4 c = conn[e];
+  Geo G;
+  element_geometry(X, c, NPE - 1, G);
+  const double kv = kappa[e] * G.meas;
+  const double m = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * (NPE == 4 ? 1.0 / 20.0 : 1.0 / 12.0);
+  double* o = ev + e * EvLayout<NPE>::kStride;
+  if (NPE == 4) {
+    double v[8];
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = a + 1; b < 4; ++b)
+        v[t++] = kv * (G.g[a][0] * G.g[b][0] + G.g[a][1] * G.g[b][1] + G.g[a][2] * G.g[b][2]);
+    v[6] = m;
+    v[7] = 0.0;
+    double4* o4 = reinterpret_cast<double4*>(o);
+    o4[0] = make_double4(v[0], v[1], v[2], v[3]);
+    o4[1] = make_double4(v[4], v[5], v[6], v[7]);
+  } else {
+    const double s01 = kv * (G.g[0][0] * G.g[1][0] + G.g[0][1] * G.g[1][1]);
+    const double s02 = kv * (G.g[0][0] * G.g[2][0] + G.g[0][1] * G.g[2][1]);
+    const double s12 = kv * (G.g[1][0] * G.g[2][0] + G.g[1][1] * G.g[2][1]);
+    *reinterpret_cast<double4*>(o) = make_double4(s01, s02, s12, m);
+  }
+}
+
+// The NPE contributions of corner a of element code (= 4 e + a) to its row:
+// columns vid[] and values val[] (the diagonal one at index a)
+template <int NPE>
+__device__ __forceinline__ void corner_row(int code, const int4* __restrict__ conn, const double* __restrict__ ev,
+                                           int vid[4], double val[4]) {
+  const int e = code >> 2, a = code & 3;
+  const int4 c = conn[e];
+  vid[0] = c.x;
+  vid[1] = c.y;
+  vid[2] = c.z;
+  vid[3] = c.w;
+  const double* r = ev + (int64_t)e * EvLayout<NPE>::kStride;
+  const double m = r[EvLayout<NPE>::kM];
+  double sum = 0.0;
+#pragma unroll
+  for (int b = 0; b < NPE; ++b) {
+    if (b == a) continue;
+    const double sab = r[EvLayout<NPE>::slot(a, b)];
+    sum += sab;
+    val[b] = sab + m;
+  }
+  val[a] = 2.0 * m - sum;
+}
+
+// rows of at most W entries: columns and sums in registers
+template <int NPE, int W>
+__global__ void __launch_bounds__(256) assemble_rows_reg_kernel(int64_t nv, int min_len,
+                                                                const int32_t* __restrict__ v2e_ptr,
+                                                                const int32_t* __restrict__ v2e,
+                                                                const int4* __restrict__ conn,
+                                                                const double* __restrict__ ev,
+                                                                const int32_t* __restrict__ rp,
+                                                                const int32_t* __restrict__ ci,
+                                                                double* __restrict__ val) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nv) return;
+  const int o = rp[row];
+  const int len = rp[row + 1] - o;
+  if (len > W || len < min_len) return;
+  int cols[W];
+  double acc[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) {
+    cols[t] = t < len ? ci[o + t] : -1;
+    acc[t] = 0.0;
+  }
+  const int pe = v2e_ptr[row + 1];
+  int p = v2e_ptr[row];
+  // two incident elements loaded at a time (independent load chains); their
+  // contributions are added in element order
+  for (; p + 1 < pe; p += 2) {
+    int vid0[4], vid1[4];
+    double v0[4], v1[4];
+    corner_row<NPE>(v2e[p], conn, ev, vid0, v0);
+    corner_row<NPE>(v2e[p + 1], conn, ev, vid1, v1);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid0[b] ? v0[b] : 0.0;
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid1[b] ? v1[b] : 0.0;
+  }
+  if (p < pe) {
+    int vid0[4];
+    double v0[4];
+    corner_row<NPE>(v2e[p], conn, ev, vid0, v0);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid0[b] ? v0[b] : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < W; ++t)
+    if (t < len) val[o + t] = acc[t];
+}
+
+__device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int c) {
   int lo = 0, hi = len;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -271,113 +422,47 @@ __device__ __forceinline__ int find_col(const int* cols, int len, int c) {
   return lo;
 }
 
-// Row-owner assembly.  acc/cols point to LDS (short rows) or to the row's own
-// slice of the global CSR arrays (long rows).
-// The 4 entries that corner a of tet e (code = 4 e + a) adds to its row:
-// values and the corners' vertex ids
-__device__ __forceinline__ void element_row(int code, int dim, int npe, bool signed_mass, double mass_coeff,
-                                            double mden, const int4* __restrict__ conn,
-                                            const double4* __restrict__ X, const double* __restrict__ kappa,
-                                            double val[4], int vid[4]) {
-  const int e = code >> 2, a = code & 3;
-  const int4 c = conn[e];
-  Geo G;
-  element_geometry(X, c, dim, G);
-  const double kv = kappa[e] * G.meas;
-  const double mv = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * mden;
-  vid[0] = c.x;
-  vid[1] = c.y;
-  vid[2] = c.z;
-  vid[3] = c.w;
-  // gradient of corner a picked by selects, and the corner loop unrolled,
-  // so G stays in registers (a dynamic index would put it in scratch)
-  double ga[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    ga[d] = a == 0 ? G.g[0][d] : (a == 1 ? G.g[1][d] : (a == 2 ? G.g[2][d] : G.g[3][d]));
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const double sp = ga[0] * G.g[b][0] + ga[1] * G.g[b][1] + ga[2] * G.g[b][2];
-    val[b] = kv * sp + mv * (a == b ? 2.0 : 1.0);
-  }
-}
-
-// Row sums over the row's incident tets in ascending element order; four
-// (then two) tets are loaded and computed at a time (independent load chains,
-// 161^3: 4.76 ms one at a time, 4.31 two, 4.15 four), their entries added in
-// order, so the sums are those of a one-at-a-time walk
-__device__ __forceinline__ void assemble_row(int64_t row, int dim, int npe, bool signed_mass, double mass_coeff,
-                                             const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
-                                             const int4* __restrict__ conn, const double4* __restrict__ X,
-                                             const double* __restrict__ kappa, const int* cols, int len,
-                                             double* acc) {
-  const double mden = (dim == 3) ? (1.0 / 20.0) : (1.0 / 12.0);
-  int p = v2e_ptr[row];
-  const int pe = v2e_ptr[row + 1];
-  for (; p + 3 < pe; p += 4) {
-    double v0[4], v1[4], v2[4], v3[4];
-    int i0[4], i1[4], i2[4], i3[4];
-    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
-    element_row(v2e[p + 1], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v1, i1);
-    element_row(v2e[p + 2], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v2, i2);
-    element_row(v2e[p + 3], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v3, i3);
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i1[b])] += v1[b];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i2[b])] += v2[b];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i3[b])] += v3[b];
-  }
-  for (; p + 1 < pe; p += 2) {
-    double v0[4], v1[4];
-    int i0[4], i1[4];
-    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
-    element_row(v2e[p + 1], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v1, i1);
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i1[b])] += v1[b];
-  }
-  if (p < pe) {
-    double v0[4];
-    int i0[4];
-    element_row(v2e[p], dim, npe, signed_mass, mass_coeff, mden, conn, X, kappa, v0, i0);
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (b < npe) acc[find_col(cols, len, i0[b])] += v0[b];
-  }
-}
-
-__global__ void __launch_bounds__(kRowBlock) assemble_rows_kernel(
-    int64_t nv, int dim, int npe, int signed_mass, double mass_coeff, const int32_t* __restrict__ v2e_ptr,
-    const int32_t* __restrict__ v2e, const int4* __restrict__ conn, const double4* __restrict__ X,
-    const double* __restrict__ kappa, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    double* __restrict__ val) {
-  __shared__ int s_cols[kRowBlock * kAsmCap];
-  __shared__ double s_acc[kRowBlock * kAsmCap];
-  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// rows longer than the register kernels take: sums in place in the CSR values
+template <int NPE>
+__global__ void __launch_bounds__(64) assemble_rows_long_kernel(int64_t nv, int min_len,
+                                                                const int32_t* __restrict__ v2e_ptr,
+                                                                const int32_t* __restrict__ v2e,
+                                                                const int4* __restrict__ conn,
+                                                                const double* __restrict__ ev,
+                                                                const int32_t* __restrict__ rp,
+                                                                const int32_t* __restrict__ ci,
+                                                                double* __restrict__ val) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nv) return;
-  int o = rp[row];
-  int len = rp[row + 1] - o;
-  if (len <= kAsmCap) {
-    int* cols = s_cols + threadIdx.x * kAsmCap;
-    double* acc = s_acc + threadIdx.x * kAsmCap;
-    for (int t = 0; t < len; ++t) { cols[t] = ci[o + t]; acc[t] = 0.0; }
-    assemble_row(row, dim, npe, signed_mass != 0, mass_coeff, v2e_ptr, v2e, conn, X, kappa, cols, len, acc);
-    for (int t = 0; t < len; ++t) val[o + t] = acc[t];
-  } else {
-    double* acc = val + o;
-    for (int t = 0; t < len; ++t) acc[t] = 0.0;
-    assemble_row(row, dim, npe, signed_mass != 0, mass_coeff, v2e_ptr, v2e, conn, X, kappa, ci + o, len, acc);
+  const int o = rp[row];
+  const int len = rp[row + 1] - o;
+  if (len < min_len) return;
+  const int32_t* cols = ci + o;
+  double* acc = val + o;
+  for (int t = 0; t < len; ++t) acc[t] = 0.0;
+  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1]; ++p) {
+    int vid[4];
+    double v[4];
+    corner_row<NPE>(v2e[p], conn, ev, vid, v);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b) acc[find_col(cols, len, vid[b])] += v[b];
   }
+}
+
+// largest row length of a CSR pattern (one atomicMax per block)
+__global__ void __launch_bounds__(256) max_row_len_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                          int* __restrict__ out) {
+  __shared__ int s[256];
+  int m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, rp[i + 1] - rp[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, s[0]);
 }
 
 // dL/dkappa_e = -|T_e| grad(a)_e . grad(u)_e, written in user element order.
@@ -625,12 +710,52 @@ void assemble_values(Handle& h) {
   hipStream_t s = h.stream;
   h.A.nslices = 0;  // the sliced-ELL copy is rebuilt with the next hierarchy
   const int npe = h.dim + 1;
+  const int stride = npe == 4 ? EvLayout<4>::kStride : EvLayout<3>::kStride;
+  h.asm_ev.reserve((size_t)h.ne * stride);
+  if (h.asm_maxlen <= 0) {
+    DevBuf<int> mx(1);
+    dev_fill_i32(mx.get(), 0, 1, s);
+    max_row_len_kernel<<<(unsigned)std::min<int64_t>(std::max<int64_t>((h.A.n + 255) / 256, 1), 1024), 256, 0, s>>>(
+        h.A.n, h.A.rp.get(), mx.get());
+    HIP_LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(&h.asm_maxlen, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  const int sm = (h.flags & 1) ? 1 : 0;
+  const int4* conn = (const int4*)h.conn.get();
+  double* ev = h.asm_ev.get();
+  const int32_t* rp = h.A.rp.get();
+  const int32_t* ci = h.A.ci.get();
+  double* val = h.A.v64.get();
+  // algorithmic bytes: connectivity, coefficient, coordinates, incidence, CSR columns and values
   ProfScope ps(h, "assembly", (double)h.ne * (16 + 8) + (double)h.nv * 32 + (double)h.A.nnz * 12 +
                                   (double)h.ne * npe * 4);
-  assemble_rows_kernel<<<grid_for(h.nv, kRowBlock), kRowBlock, 0, s>>>(
-      h.nv, h.dim, npe, (h.flags & 1) ? 1 : 0, h.mass_coeff, h.v2e_ptr.get(), h.v2e.get(),
-      (const int4*)h.conn.get(), (const double4*)h.coords.get(), h.kappa.get(), h.A.rp.get(), h.A.ci.get(),
-      h.A.v64.get());
+  const unsigned ge = grid_for(h.ne, 256), gr = grid_for(h.nv, 256);
+  if (npe == 4) {
+    element_values_kernel<4><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
+                                                h.kappa.get(), ev);
+    HIP_LAUNCH_CHECK();
+    assemble_rows_reg_kernel<4, 16><<<gr, 256, 0, s>>>(h.nv, 0, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
+                                                       val);
+    if (h.asm_maxlen > 16)
+      assemble_rows_reg_kernel<4, 32><<<gr, 256, 0, s>>>(h.nv, 17, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
+                                                         val);
+    if (h.asm_maxlen > 32)
+      assemble_rows_long_kernel<4><<<grid_for(h.nv, 64), 64, 0, s>>>(h.nv, 33, h.v2e_ptr.get(), h.v2e.get(), conn,
+                                                                     ev, rp, ci, val);
+  } else {
+    element_values_kernel<3><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
+                                                h.kappa.get(), ev);
+    HIP_LAUNCH_CHECK();
+    assemble_rows_reg_kernel<3, 16><<<gr, 256, 0, s>>>(h.nv, 0, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
+                                                       val);
+    if (h.asm_maxlen > 16)
+      assemble_rows_reg_kernel<3, 32><<<gr, 256, 0, s>>>(h.nv, 17, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
+                                                         val);
+    if (h.asm_maxlen > 32)
+      assemble_rows_long_kernel<3><<<grid_for(h.nv, 64), 64, 0, s>>>(h.nv, 33, h.v2e_ptr.get(), h.v2e.get(), conn,
+                                                                     ev, rp, ci, val);
+  }
   HIP_LAUNCH_CHECK();
 }
 

@@ -143,6 +143,12 @@ struct Handle {
   bool has_mesh = false;
   bool matrix_from_file = false;
 
+  // element-centric assembly: per-element records (off-diagonal stiffness
+  // entries and the mass value, 8 doubles per tet / 4 per triangle), kept
+  // for re-assembly; largest row length of the pattern (0: not yet known)
+  DevBuf<double> asm_ev;
+  int asm_maxlen = 0;
+
   // level-0 operator (FP64, internal numbering)
   Csr A;
   std::vector<int64_t> mesh_pattern_rp;  // user-numbered mesh pattern (host)

@@ -128,3 +128,83 @@ def test_invalid_mesh_inputs_rejected(fem, gpu, meshgen):
         s.update_kappa(bad_k)
     assert e.value.status == 6
     s.close()
+
+
+def _label_table(labels):
+    """Material label -> coefficient of perform_element_loop_3D.cuh:585-610:
+    labels 0 and 1 give 1, labels 2..6 give the label itself."""
+    co = np.ones(labels.shape, dtype=np.float64)
+    hi = (labels >= 2) & (labels <= 6)
+    co[hi] = labels[hi]
+    return co
+
+
+def test_labelled_tetgen_file_matches_oracle(fem, gpu, oracle, meshgen, tmp_path):
+    """A TetGen mesh whose .ele carries material labels 0..6 (1-based
+    indices, as TetGen writes them), through the file constructor: the
+    element coefficients follow the reference's label table and the mass
+    coefficient is 1; coordinates are parsed as float as the reference does."""
+    report("[gpu] labelled TetGen file (labels 0..6)")
+    xyz, tets = meshgen.cube_mesh(6, seed=21)
+    rng = np.random.default_rng(21)
+    labels = rng.integers(0, 7, size=tets.shape[0])
+    assert set(np.unique(labels)) == set(range(7))
+    base = tmp_path / "labelled"
+    with open(str(base) + ".node", "w") as f:
+        f.write(f"{xyz.shape[0]} 3 0 0\n")
+        for i, p in enumerate(xyz):
+            # FP32 values written exactly (9 significant digits identify a float)
+            q = p.astype(np.float32)
+            f.write(f"{i + 1} {q[0]:.9g} {q[1]:.9g} {q[2]:.9g}\n")
+    with open(str(base) + ".ele", "w") as f:
+        f.write(f"{tets.shape[0]} 4 1\n")
+        for e, t in enumerate(tets):
+            f.write(f"{e + 1} {t[0] + 1} {t[1] + 1} {t[2] + 1} {t[3] + 1} {labels[e]}\n")
+    xr, tr, lr = fem.read_tetgen(str(base))
+    assert np.array_equal(tr, tets) and np.array_equal(lr, labels)
+    assert np.array_equal(xr, xyz.astype(np.float32).astype(np.float64))
+    s = fem.FEMSolver(str(base), True, False)
+    rp, ci, v = s.get_matrix_csr()
+    orp, oci, ov = oracle.assemble(xr, tets, _label_table(labels), 1.0)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert np.abs(v - ov).max() <= 1e-13 * np.abs(ov).max()
+    # the same mesh with every label 1 differs: the coefficients were applied
+    orp1, _, ov1 = oracle.assemble(xr, tets, None, 1.0)
+    assert np.abs(ov1 - ov).max() > 0.1 * np.abs(ov).max()
+    s.close()
+
+
+def test_signed_mass_2d_matches_oracle(fem, gpu, oracle, meshgen, tmp_path):
+    """2D mass with the signed Jacobian (perform_element_loop_2D.cuh:92):
+    clockwise triangles get negative mass.  Through fem_create_from_arrays
+    with flags bit 0 and through a triangle-mesh file (OFF), against the
+    oracle's signed element matrices; without the flag the mass is unsigned."""
+    report("[gpu] 2D signed mass")
+    xyz, tris = meshgen.square_mesh(9, seed=5)
+    rng = np.random.default_rng(5)
+    flip = rng.random(tris.shape[0]) < 0.3
+    tris = tris.copy()
+    tris[flip] = tris[flip][:, [0, 2, 1]]  # clockwise
+    for signed in (True, False):
+        s, rp, ci, v = _gpu_matrix(fem, 2, xyz, tris, None, 1.0, flags=1 if signed else 0)
+        orp, oci, ov = oracle.assemble(xyz, tris, None, 1.0, signed_mass=signed)
+        assert np.array_equal(ci, oci)
+        assert np.abs(v - ov).max() <= 1e-13 * np.abs(ov).max()
+        s.close()
+    orp, oci, ov_s = oracle.assemble(xyz, tris, None, 1.0, signed_mass=True)
+    _, _, ov_u = oracle.assemble(xyz, tris, None, 1.0, signed_mass=False)
+    assert np.abs(ov_s - ov_u).max() > 1e-6  # the flag matters on this mesh
+    # file path: TriMesh-style float points, signed mass, mass coefficient 1
+    p = tmp_path / "flip.off"
+    lines = ["OFF", f"{xyz.shape[0]} {tris.shape[0]} 0"]
+    lines += [f"{q[0]:.9g} {q[1]:.9g} 0" for q in xyz.astype(np.float32)]
+    lines += ["3 " + " ".join(str(int(i)) for i in t) for t in tris]
+    p.write_text("\n".join(lines) + "\n")
+    s = fem.FEMSolver(str(p), False, False)
+    rp, ci, v = s.get_matrix_csr()
+    xf = xyz.astype(np.float32).astype(np.float64)
+    xf[:, 2] = 0.0
+    orp, oci, ov = oracle.assemble(xf, tris, None, 1.0, signed_mass=True)
+    assert np.array_equal(ci, oci)
+    assert np.abs(v - ov).max() <= 1e-13 * np.abs(ov).max()
+    s.close()

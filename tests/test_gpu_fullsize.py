@@ -7,17 +7,22 @@ against a host CSR product, and every level matrix through the V-cycle's
 kernels against host products."""
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import report
+from conftest import ROOT, report
 
 pytestmark = pytest.mark.gpu
 
-# bench.py's V-cycle settings
-PARAMS = dict(solverType=1, tolerance=1e-8, maxIters=500, smootherType=1, preInnerIters=3, postInnerIters=3,
-              postRelaxes=1, topSize=256, aggregationRule=1, strengthThreshold=0.005)
+sys.path.insert(0, str(ROOT))
+import bench  # noqa: E402  (the benchmark's V-cycle settings are the tested ones)
+
+# bench.py's V-cycle settings (precision set per test)
+PARAMS = {k: v for k, v in bench.solver_params(1).items() if k != "precision"}
+assert PARAMS["strengthThreshold"] == 0.01
 
 
 def _config(meshgen, name):
@@ -182,10 +187,35 @@ def test_c1_square65(fem, gpu, oracle, meshgen):
     s.close()
 
 
+def _secant_fd(s, sd, kappa, d, h, b, u_obs):
+    """Central difference quotient (L(kappa + h d) - L(kappa - h d)) / (2 h) of
+    L = 0.5 |u - u_obs|^2, evaluated through the exact secant identity instead
+    of subtracting two nearly equal losses: with K(kappa) = sum kappa_e S_e + M,
+    K_+ - K_- = 2 h S(d), so u_+ - u_- = -2 h w with w = K_+^-1 S(d) u_- and
+        (L_+ - L_-) / (2 h) = -w.(u_- - u_obs) + h w.w
+    exactly.  The solves' errors then enter relative to w, not to the 1e-6-sized
+    difference of two losses.  sd: a handle on the same mesh assembled with
+    kappa = d and no mass (S(d) u by its operator)."""
+    s.update_kappa(kappa - h * d)
+    um, im = s.solve(b)
+    assert im["status"] == 0
+    sd.update_kappa(d)
+    rhs = sd.apply_operator(um)
+    s.update_kappa(kappa + h * d)
+    w, iw = s.solve(rhs)
+    assert iw["status"] == 0
+    return float(-np.dot(w, um - u_obs) + h * np.dot(w, w))
+
+
 def test_c4_adjoint_full_size(fem, gpu, meshgen):
     """C4: 49^3, sigma 1, L = 0.5 |u - u_obs|^2 with u_obs from a second seeded
-    coefficient plus N(0, 1e-3) noise; the adjoint gradient against central
-    differences of L (GPU FP64 solves at 1e-12) along 4 seeded directions."""
+    coefficient plus N(0, 1e-3) noise.  The GPU adjoint gradient against
+    (1) the FP64 CPU reference gradient (SuperLU solves of the oracle's
+    matrices and the oracle's element gradient, tests/golden/c4_gradient.npz
+    from tests/golden/make_c4_gradient.py) at 4096 seeded elements and in
+    norm; (2) central differences along 16 seeded directions at h = 1e-6
+    (exact secant identity, GPU FP64 solves at 1e-12); (3) plain central
+    differences of the loss along 4 directions at h = 1e-4."""
     report("[gpu] full-size c4_cube49_adjoint")
     xyz, tets, kappa, mass = _config(meshgen, "c4_cube49_adjoint")
     kappa2 = meshgen.lognormal_kappa(101, tets.shape[0], 1.0)
@@ -201,8 +231,30 @@ def test_c4_adjoint_full_size(fem, gpu, meshgen):
     u, info = s.solve(b)
     g, _, ainfo = s.adjoint_grad(u, u - u_obs)
     assert info["status"] == 0 and ainfo["status"] == 0
+    # (1) FP64 CPU reference gradient
+    fx = np.load(ROOT / "tests" / "golden" / "c4_gradient.npz")
+    idx, g_ref = fx["idx"], fx["g"]
+    err_s = np.linalg.norm(g[idx] - g_ref) / np.linalg.norm(g_ref)
+    err_n = abs(np.linalg.norm(g) - float(fx["g_norm"])) / float(fx["g_norm"])
+    loss = 0.5 * float(np.dot(u - u_obs, u - u_obs))
+    report(f"      vs FP64 CPU gradient: sampled rel error {err_s:.2e}, norm rel error {err_n:.2e}, "
+           f"loss rel error {abs(loss - float(fx['loss'])) / float(fx['loss']):.2e}")
+    assert err_s <= 1e-7 and err_n <= 1e-8
+    # (2) 16 directions, h = 1e-6
+    sd = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=np.ones_like(kappa), mass_coeff=0.0)
+    rng = np.random.default_rng(1649)
+    errs = []
+    for _ in range(16):
+        d = rng.normal(size=kappa.size) * kappa
+        fd = _secant_fd(s, sd, kappa, d, 1e-6, b, u_obs)
+        ad = float(np.dot(g, d))
+        errs.append(abs(fd - ad) / (np.linalg.norm(g) * np.linalg.norm(d)))
+    report(f"      16 directions h=1e-6: max scaled error {max(errs):.1e}")
+    assert max(errs) < 1e-7
+    sd.close()
 
-    def loss(k):
+    # (3) plain central differences of the loss, h = 1e-4
+    def loss_at(k):
         s.update_kappa(k)
         uk, ik = s.solve(b)
         assert ik["status"] == 0
@@ -212,9 +264,62 @@ def test_c4_adjoint_full_size(fem, gpu, meshgen):
     h = 1e-4
     for _ in range(4):
         d = rng.normal(size=kappa.size) * kappa
-        fd = (loss(kappa + h * d) - loss(kappa - h * d)) / (2 * h)
+        fd = (loss_at(kappa + h * d) - loss_at(kappa - h * d)) / (2 * h)
         ad = float(np.dot(g, d))
         err = abs(fd - ad) / (np.linalg.norm(g) * np.linalg.norm(d))
         report(f"      FD {fd:.6e} adjoint {ad:.6e} scaled error {err:.1e}")
         assert err < 1e-6
+    s.close()
+
+
+def test_c5_batched_and_adjoint_full_size(fem, gpu, meshgen):
+    """C5's defining legs at 161^3: the 16 right-hand sides through one batched
+    V-cycle (true FP64 residual <= 1e-8 per column on the host, and two
+    columns against their single-right-hand-side solves), and the adjoint
+    gradient for one right-hand side against central differences along 2
+    seeded directions (h = 1e-6, exact secant identity) on the mixed path."""
+    report("[gpu] full-size c5_cube161_mixed16: 16-RHS solve and adjoint")
+    xyz, tets, kappa, mass = _config(meshgen, "c5_cube161_mixed16")
+    n = xyz.shape[0]
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=mass)
+    s.set_params(precision=1, **PARAMS)
+    A = s.get_matrix_scipy()
+    B = meshgen.gaussian_rhs(0, n, 16)
+    X, info = s.solve(B)
+    assert info["status"] == 0
+    R = B - A @ X
+    res = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    report(f"      16 RHS: {info['iterations']} iterations, max column residual {res.max():.2e}")
+    assert np.all(res <= 1e-8 * (1 + 1e-6))
+    for c in (0, 11):
+        x1, i1 = s.solve(np.ascontiguousarray(B[:, c]))
+        assert i1["status"] == 0
+        diff = np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1)
+        report(f"      column {c}: |x_batched - x_single| / |x_single| = {diff:.2e}")
+        assert diff <= 1e-6
+    del R, X
+    # adjoint for one right-hand side (tolerance 1e-10 so the FD quotients'
+    # solve errors stay far below the gradient)
+    b = np.ascontiguousarray(B[:, 0])
+    del B
+    s.set_params(precision=1, **dict(PARAMS, tolerance=1e-10, maxIters=1000))
+    u, info = s.solve(b)
+    # dL/du = -noise: a rough right-hand side (a smooth one is dominated by the
+    # lowest, mass-scaled modes, whose FP64 residual floor on this 7e9-condition
+    # operator sits above 1e-10)
+    u_obs = u + meshgen.noise(5, n, 1e-3)
+    g, _, ainfo = s.adjoint_grad(u, u - u_obs)
+    report(f"      forward {info['iterations']} it (res {info['rel_residual']:.1e}), adjoint "
+           f"{ainfo['iterations']} it (res {ainfo['rel_residual']:.1e})")
+    assert info["status"] == 0 and ainfo["status"] == 0
+    sd = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=np.ones_like(kappa), mass_coeff=0.0)
+    rng = np.random.default_rng(161)
+    for _ in range(2):
+        d = rng.normal(size=kappa.size) * kappa
+        fd = _secant_fd(s, sd, kappa, d, 1e-6, b, u_obs)
+        ad = float(np.dot(g, d))
+        err = abs(fd - ad) / (np.linalg.norm(g) * np.linalg.norm(d))
+        report(f"      FD {fd:.6e} adjoint {ad:.6e} scaled error {err:.1e}")
+        assert err < 1e-6
+    sd.close()
     s.close()

@@ -132,17 +132,21 @@ def test_coarse_solve_in_fp64(fem, gpu, meshgen, precision):
     s.close()
 
 
-@pytest.mark.parametrize("precision,smoother,N,tol", [
-    (0, 1, 21, 1e-12), (0, 0, 21, 1e-12), (1, 1, 21, 1e-4), (1, 0, 21, 1e-4), (2, 1, 21, 2e-2),
-    (1, 1, 97, 1e-4)])
-def test_preconditioner_matches_host_vcycle(fem, gpu, meshgen, precision, smoother, N, tol):
+@pytest.mark.parametrize("precision,smoother,N", [(0, 1, 21), (0, 0, 21), (1, 1, 21), (1, 0, 21), (2, 1, 21),
+                                                (1, 1, 97)])
+def test_preconditioner_matches_host_vcycle(fem, gpu, meshgen, precision, smoother, N):
     """z = M r through fem_apply_preconditioner (mixed one-column path: the
     first level-0 step reads the FP64 r, the last writes z in FP64) against
-    the FP64 host V-cycle on the same hierarchy.  Relative 2-norm bounds:
-    FP64 1e-12; FP32 hierarchy 1e-4 (FP32 rounding of ~10 sparse passes with
-    cancellation in the residuals; a wrong coefficient or a dropped term is
-    O(0.1)); FP16 level-0 smoother matrix (precision 2) 2e-2 (off-diagonals
-    rounded to 11 bits)."""
+    the FP64 host V-cycle on the same hierarchy, for a random r and for a
+    smooth r = A v.  Relative 2-norm bounds:
+    - FP64 hierarchy: 1e-10 (FP64 rounding, amplified by the coarse solve);
+    - FP32 hierarchy: 20 e_in + 1e-6, where e_in = |M r - M fl32(r)| / |M r|
+      is what rounding the input alone to FP32 changes in the host V-cycle
+      (the kernels round ~10 intermediate vectors the same way; for smooth r
+      the coarse correction amplifies such rounding to ~1e-4..1e-3);
+    - FP16 level-0 smoother matrix (precision 2): 2e-2 (off-diagonals rounded
+      to 11 bits).
+    A wrong smoothing coefficient or a dropped term changes z by O(0.1)."""
     report(f"[gpu] preconditioner vs host V-cycle, precision={precision} smoother={smoother} N={N}")
     sigma = 2.0 if N > 21 else 1.5
     s = _solver(fem, meshgen, N, sigma, precision, smoother, topSize=256 if N > 21 else 64)
@@ -155,11 +159,16 @@ def test_preconditioner_matches_host_vcycle(fem, gpu, meshgen, precision, smooth
         if trial == 1:
             r_user = r_user[np.argsort(perm)]  # internal -> user order
         z_user = s.apply_preconditioner(r_user)
-        z_int = host.vcycle(0, r_user[perm])
+        r_int = r_user[perm]
+        z_int = host.vcycle(0, r_int)
         z_ref = np.empty(n)
         z_ref[perm] = z_int
         err = np.linalg.norm(z_user - z_ref) / np.linalg.norm(z_ref)
-        report(f"      trial {trial}: {s.num_levels()} levels, rel error {err:.2e}")
+        e_in = np.linalg.norm(host.vcycle(0, r_int.astype(np.float32).astype(np.float64)) - z_int) / \
+            np.linalg.norm(z_int)
+        tol = {0: 1e-10, 1: 20 * e_in + 1e-6, 2: 2e-2}[precision]
+        report(f"      trial {trial}: {s.num_levels()} levels, rel error {err:.2e} (input-rounding effect "
+               f"{e_in:.1e}, bound {tol:.1e})")
         assert err <= tol
     s.close()
 
