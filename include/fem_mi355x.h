@@ -59,7 +59,11 @@ typedef struct fem_params {
   int aggregatorType;     /* aggregatorType_     0  (only 0 is implemented)   */
   int convergeType;       /* convergeType_       0  accepted, no effect       */
   double tolerance;       /* tolerance_          1e-6 relative residual       */
-  int cycleType;          /* cycleType_          0  V only                    */
+  int cycleType;          /* cycleType_          0  V; 1 W (coarse correction:
+                             two W-cycles of the next level); 2 F (an F-cycle
+                             then a V-cycle of the next level; the PCG then
+                             takes the flexible beta); 3 K: a V-cycle (the
+                             reference runs K as its outer CG)                */
   int solverType;         /* solverType_         0: one V-cycle, 1: PCG       */
   double smootherWeight;  /* smootherWeight_     1.0                          */
   double proOmega;        /* proOmega_           0.67 prolongator smoothing   */

@@ -16,6 +16,7 @@
 // summation order (bit-reproducible), and A is bitwise symmetric because both
 // entries of a pair add the same stored element values in the same order.
 #include "fem_internal.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -271,15 +272,20 @@ __device__ __forceinline__ void element_geometry(const double4* __restrict__ X, 
 // The element matrix is then K_ab = S_ab + m (a != b) and, because the
 // stiffness rows of a P1 element sum to zero, K_aa = -sum_{b != a} S_ab + 2 m.
 //
-// Pass 2 (assemble_rows_*): one thread per matrix row walks the elements
-// incident to its vertex in ascending element order and adds the row of each
-// element matrix into the row's entries.  Every entry is a sum over its
-// elements in ascending element order, and entry (i, j) and entry (j, i) add
-// the same stored S_ab + m, so A is deterministic and bitwise symmetric
-// without float atomics.  Rows of up to W entries keep their columns and
-// sums in registers (a contribution is added to the slot whose column
-// matches: compare-and-select over the W slots, no LDS and no search);
-// longer rows sum in place in the CSR value array.
+// Pass 2 (assemble_slices_kernel): a wavefront owns a slice of 64 consecutive
+// rows; each lane walks the elements incident to its row in ascending element
+// order and adds the row of each element matrix into the row's entries.
+// Every entry is a sum over its elements in ascending element order, and
+// entry (i, j) and entry (j, i) add the same stored S_ab + m, so A is
+// deterministic and bitwise symmetric without float atomics.
+// The incidences are stored per slice, lane-interleaved ([k][64]: incidence k
+// of the slice's 64 rows is one coalesced 512-byte read), each with the
+// positions of the element's vertices in the row (one byte each), so no
+// column search runs at assembly time.  The slice's rows are contiguous in
+// the CSR value array: they are summed in LDS at their CSR offsets and
+// written back by the wavefront as one coalesced run.  Slices whose rows
+// span more than kSliceSpan entries, or hold a row longer than 255 entries,
+// sum in place in the CSR values with a column search (any row length).
 
 template <int NPE>
 struct EvLayout;
@@ -306,7 +312,9 @@ __global__ void __launch_bounds__(256) element_values_kernel(int64_t ne, int sig
                                                              const double4* __restrict__ X,
                                                              const double* __restrict__ kappa,
                                                              double* __restrict__ ev) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous block order: neighbouring elements share vertices, and
+  // each XCD's L2 then reads a contiguous range of the coordinates
+  const int64_t e = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int4 c = conn[e];
   Geo G;
@@ -335,18 +343,10 @@ __global__ void __launch_bounds__(256) element_values_kernel(int64_t ne, int sig
   }
 }
 
-// The NPE contributions of corner a of element code (= 4 e + a) to its row:
-// columns vid[] and values val[] (the diagonal one at index a)
+// The NPE contributions of corner a of an element record r to its row:
+// val[b] = S_ab + m (b != a), val[a] = 2 m - sum_{b != a} S_ab
 template <int NPE>
-__device__ __forceinline__ void corner_row(int code, const int4* __restrict__ conn, const double* __restrict__ ev,
-                                           int vid[4], double val[4]) {
-  const int e = code >> 2, a = code & 3;
-  const int4 c = conn[e];
-  vid[0] = c.x;
-  vid[1] = c.y;
-  vid[2] = c.z;
-  vid[3] = c.w;
-  const double* r = ev + (int64_t)e * EvLayout<NPE>::kStride;
+__device__ __forceinline__ void corner_values(int a, const double* r, double val[4]) {
   const double m = r[EvLayout<NPE>::kM];
   double sum = 0.0;
 #pragma unroll
@@ -359,58 +359,169 @@ __device__ __forceinline__ void corner_row(int code, const int4* __restrict__ co
   val[a] = 2.0 * m - sum;
 }
 
-// rows of at most W entries: columns and sums in registers
-template <int NPE, int W>
-__global__ void __launch_bounds__(256) assemble_rows_reg_kernel(int64_t nv, int min_len,
-                                                                const int32_t* __restrict__ v2e_ptr,
-                                                                const int32_t* __restrict__ v2e,
-                                                                const int4* __restrict__ conn,
-                                                                const double* __restrict__ ev,
-                                                                const int32_t* __restrict__ rp,
-                                                                const int32_t* __restrict__ ci,
-                                                                double* __restrict__ val) {
+// element record of element e (8 doubles for tets, 4 for triangles)
+template <int NPE>
+__device__ __forceinline__ void load_record(const double* __restrict__ ev, int e, double r[8]) {
+  const double4* p = reinterpret_cast<const double4*>(ev + (int64_t)e * EvLayout<NPE>::kStride);
+  const double4 a = p[0];
+  r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+  if (NPE == 4) {
+    const double4 b = p[1];
+    r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+  }
+}
+
+// The NPE contributions of corner a of element code (= 4 e + a) to its row:
+// columns vid[] and values val[] (the diagonal one at index a)
+template <int NPE>
+__device__ __forceinline__ void corner_row(int code, const int4* __restrict__ conn, const double* __restrict__ ev,
+                                           int vid[4], double val[4]) {
+  const int e = code >> 2, a = code & 3;
+  const int4 c = conn[e];
+  vid[0] = c.x;
+  vid[1] = c.y;
+  vid[2] = c.z;
+  vid[3] = c.w;
+  double r[8];
+  load_record<NPE>(ev, e, r);
+  corner_values<NPE>(a, r, val);
+}
+
+constexpr int kSliceSpan = 2048;  // LDS doubles per wavefront (16 KiB)
+
+// per slice of 64 rows: the most incidences of a row (0 for a slice that
+// takes the in-place path) and whether the slice takes that path
+__global__ void inc_slice_kernel(int64_t nv, const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ rp,
+                                 int32_t* __restrict__ kmax, int32_t* __restrict__ span, int32_t* __restrict__ is_long) {
+  const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nsl = (nv + 63) / 64;
+  if (sl >= nsl) return;
+  const int64_t r0 = sl * 64, r1 = r0 + 64 < nv ? r0 + 64 : nv;
+  int km = 0, ml = 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    km = max(km, v2e_ptr[r + 1] - v2e_ptr[r]);
+    ml = max(ml, rp[r + 1] - rp[r]);
+  }
+  const int sp = rp[r1] - rp[r0];
+  const int lg = (sp > kSliceSpan || ml > 255) ? 1 : 0;
+  kmax[sl] = lg ? 0 : km;
+  span[sl] = sp;
+  is_long[sl] = lg;
+}
+
+// lane-interleaved incidences of the slices that sum in LDS: (element code,
+// positions of the element's vertices in the row, one byte each); code -1
+// pads a row's list to the slice's longest
+template <int NPE>
+__global__ void inc_fill_kernel(int64_t nv, const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
+                                const int4* __restrict__ conn, const int32_t* __restrict__ rp,
+                                const int32_t* __restrict__ ci, const int32_t* __restrict__ sptr,
+                                int2* __restrict__ inc) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nv) return;
-  const int o = rp[row];
-  const int len = rp[row + 1] - o;
-  if (len > W || len < min_len) return;
-  int cols[W];
-  double acc[W];
+  const int64_t sl = row >> 6;
+  const int lane = (int)(row & 63);
+  const int k0 = sptr[sl], kn = sptr[sl + 1] - k0;
+  if (kn == 0) return;
+  const int o = rp[row], len = rp[row + 1] - o;
+  const int p0 = v2e_ptr[row], cnt = v2e_ptr[row + 1] - p0;
+  for (int k = 0; k < kn; ++k) {
+    int2 out = make_int2(-1, 0);
+    if (k < cnt) {
+      const int code = v2e[p0 + k];
+      const int4 c = conn[code >> 2];
+      const int vid[4] = {c.x, c.y, c.z, c.w};
+      unsigned slots = 0;
 #pragma unroll
-  for (int t = 0; t < W; ++t) {
-    cols[t] = t < len ? ci[o + t] : -1;
-    acc[t] = 0.0;
+      for (int b = 0; b < NPE; ++b) {
+        int lo = 0, hi = len;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (ci[o + mid] < vid[b]) lo = mid + 1; else hi = mid;
+        }
+        slots |= (unsigned)lo << (8 * b);
+      }
+      out = make_int2(code, (int)slots);
+    }
+    inc[((int64_t)(k0 + k) << 6) + lane] = out;
   }
-  const int pe = v2e_ptr[row + 1];
-  int p = v2e_ptr[row];
-  // two incident elements loaded at a time (independent load chains); their
-  // contributions are added in element order
-  for (; p + 1 < pe; p += 2) {
-    int vid0[4], vid1[4];
-    double v0[4], v1[4];
-    corner_row<NPE>(v2e[p], conn, ev, vid0, v0);
-    corner_row<NPE>(v2e[p + 1], conn, ev, vid1, v1);
-#pragma unroll
-    for (int b = 0; b < NPE; ++b)
-#pragma unroll
-      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid0[b] ? v0[b] : 0.0;
-#pragma unroll
-    for (int b = 0; b < NPE; ++b)
-#pragma unroll
-      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid1[b] ? v1[b] : 0.0;
+}
+
+// rows of the slices that sum in LDS (see above)
+template <int NPE>
+__global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const int32_t* __restrict__ sptr,
+                                                              const int32_t* __restrict__ is_long,
+                                                              const int2* __restrict__ inc,
+                                                              const double* __restrict__ ev,
+                                                              const int32_t* __restrict__ rp,
+                                                              double* __restrict__ val, int span_max) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nsl = (nv + 63) / 64;
+  // XCD-contiguous block order: an element record serves the rows of its
+  // four vertices, which lie in the same or nearby slices, so each XCD's L2
+  // reads a contiguous range of records once
+  const int64_t sl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+  double* acc = lds + wave * span_max;
+  // own: this wavefront sums its slice here (span <= span_max, rows <= 255)
+  const bool own = sl < nsl && is_long[sl] == 0;
+  int kn = 0, base = 0, span = 0;
+  int64_t o0 = 0;
+  if (own) {
+    const int64_t r0 = sl * 64, r1 = r0 + 64 < nv ? r0 + 64 : nv;
+    o0 = rp[r0];
+    span = rp[r1] - (int)o0;
+    if (r0 + lane < nv) {  // lanes past the last row take no incidences
+      kn = sptr[sl + 1] - sptr[sl];
+      base = rp[r0 + lane] - (int)o0;
+    }
+    for (int t = lane; t < span; t += 64) acc[t] = 0.0;
   }
-  if (p < pe) {
-    int vid0[4];
-    double v0[4];
-    corner_row<NPE>(v2e[p], conn, ev, vid0, v0);
+  __syncthreads();
+  if (kn > 0) {
+    const int2* q = inc + ((int64_t)sptr[sl] << 6) + lane;
+    // the diagonal sums in a register (every incidence adds to it), the
+    // off-diagonal entries in LDS; both in incidence order
+    double diag = 0.0;
+    int dslot = -1;
+    auto add = [&](int2 ic, const double* r) {
+      const int a = ic.x & 3;
+      double v[4];
+      corner_values<NPE>(a, r, v);
 #pragma unroll
-    for (int b = 0; b < NPE; ++b)
-#pragma unroll
-      for (int t = 0; t < W; ++t) acc[t] += cols[t] == vid0[b] ? v0[b] : 0.0;
+      for (int b = 0; b < NPE; ++b) {
+        const int sb = (ic.y >> (8 * b)) & 255;
+        if (b == a) {
+          diag += v[b];
+          dslot = sb;
+        } else {
+          acc[base + sb] += v[b];
+        }
+      }
+    };
+    int k = 0;
+    // two incidences loaded at a time (independent load chains), added in order
+    for (; k + 1 < kn; k += 2) {
+      const int2 i0 = q[(int64_t)k << 6], i1 = q[(int64_t)(k + 1) << 6];
+      double r0[8], r1[8];
+      if (i0.x >= 0) load_record<NPE>(ev, i0.x >> 2, r0);
+      if (i1.x >= 0) load_record<NPE>(ev, i1.x >> 2, r1);
+      if (i0.x >= 0) add(i0, r0);
+      if (i1.x >= 0) add(i1, r1);
+    }
+    if (k < kn) {
+      const int2 i0 = q[(int64_t)k << 6];
+      if (i0.x >= 0) {
+        double r0[8];
+        load_record<NPE>(ev, i0.x >> 2, r0);
+        add(i0, r0);
+      }
+    }
+    if (dslot >= 0) acc[base + dslot] = diag;
   }
-#pragma unroll
-  for (int t = 0; t < W; ++t)
-    if (t < len) val[o + t] = acc[t];
+  __syncthreads();
+  if (own)
+    for (int t = lane; t < span; t += 64) val[o0 + t] = acc[t];
 }
 
 __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int c) {
@@ -422,9 +533,11 @@ __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int le
   return lo;
 }
 
-// rows longer than the register kernels take: sums in place in the CSR values
+// rows of the slices listed in long_sl: sums in place in the CSR values,
+// columns found by binary search (any row length)
 template <int NPE>
-__global__ void __launch_bounds__(64) assemble_rows_long_kernel(int64_t nv, int min_len,
+__global__ void __launch_bounds__(64) assemble_rows_long_kernel(int64_t nv, int64_t nlong,
+                                                                const int32_t* __restrict__ long_sl,
                                                                 const int32_t* __restrict__ v2e_ptr,
                                                                 const int32_t* __restrict__ v2e,
                                                                 const int4* __restrict__ conn,
@@ -432,11 +545,11 @@ __global__ void __launch_bounds__(64) assemble_rows_long_kernel(int64_t nv, int 
                                                                 const int32_t* __restrict__ rp,
                                                                 const int32_t* __restrict__ ci,
                                                                 double* __restrict__ val) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int64_t)blockIdx.x >= nlong) return;
+  const int64_t row = (int64_t)long_sl[blockIdx.x] * 64 + threadIdx.x;
   if (row >= nv) return;
   const int o = rp[row];
   const int len = rp[row + 1] - o;
-  if (len < min_len) return;
   const int32_t* cols = ci + o;
   double* acc = val + o;
   for (int t = 0; t < len; ++t) acc[t] = 0.0;
@@ -447,22 +560,6 @@ __global__ void __launch_bounds__(64) assemble_rows_long_kernel(int64_t nv, int 
 #pragma unroll
     for (int b = 0; b < NPE; ++b) acc[find_col(cols, len, vid[b])] += v[b];
   }
-}
-
-// largest row length of a CSR pattern (one atomicMax per block)
-__global__ void __launch_bounds__(256) max_row_len_kernel(int64_t n, const int32_t* __restrict__ rp,
-                                                          int* __restrict__ out) {
-  __shared__ int s[256];
-  int m = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    m = max(m, rp[i + 1] - rp[i]);
-  s[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(out, s[0]);
 }
 
 // dL/dkappa_e = -|T_e| grad(a)_e . grad(u)_e, written in user element order.
@@ -706,21 +803,54 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
   assemble_values(h);
 }
 
+// Slice layout of the incidences for assemble_slices_kernel (once per
+// pattern; re-assembly with new coefficients reuses it).
+void build_assembly_slices(Handle& h) {
+  hipStream_t s = h.stream;
+  const int64_t nv = h.nv, nsl = (nv + 63) / 64;
+  DevBuf<int32_t> kmax(nsl), span(nsl), lg(nsl);
+  inc_slice_kernel<<<grid_for(nsl, 256), 256, 0, s>>>(nv, h.v2e_ptr.get(), h.A.rp.get(), kmax.get(), span.get(),
+                                                      lg.get());
+  HIP_LAUNCH_CHECK();
+  h.asm_sptr.resize(nsl + 1);
+  const int64_t tot = dev_exclusive_scan_i32(kmax.get(), h.asm_sptr.get(), (size_t)nsl, s);
+  if (tot * 64 >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "too many element incidences for the assembly layout");
+  std::vector<int32_t> hs(nsl), hl(nsl);
+  span.download(hs.data(), (size_t)nsl, s);
+  lg.download(hl.data(), (size_t)nsl, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  int smax = 1;
+  std::vector<int32_t> longs;
+  for (int64_t i = 0; i < nsl; ++i) {
+    if (hl[i]) longs.push_back((int32_t)i);
+    else smax = std::max(smax, hs[i]);
+  }
+  h.asm_span = smax;
+  h.asm_nlong = (int64_t)longs.size();
+  h.asm_long.resize(longs.empty() ? 1 : longs.size());
+  if (!longs.empty()) h.asm_long.upload(longs.data(), longs.size(), s);
+  h.asm_islong = std::move(lg);
+  h.asm_inc.resize(tot > 0 ? (size_t)tot * 64 : 1);
+  dev_fill_i32(reinterpret_cast<int32_t*>(h.asm_inc.get()), -1, 2 * h.asm_inc.size(), s);
+  const int4* conn = (const int4*)h.conn.get();
+  if (h.dim == 3)
+    inc_fill_kernel<4><<<grid_for(nv, 256), 256, 0, s>>>(nv, h.v2e_ptr.get(), h.v2e.get(), conn, h.A.rp.get(),
+                                                         h.A.ci.get(), h.asm_sptr.get(), h.asm_inc.get());
+  else
+    inc_fill_kernel<3><<<grid_for(nv, 256), 256, 0, s>>>(nv, h.v2e_ptr.get(), h.v2e.get(), conn, h.A.rp.get(),
+                                                         h.A.ci.get(), h.asm_sptr.get(), h.asm_inc.get());
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipStreamSynchronize(s));
+  h.asm_ready = true;
+}
+
 void assemble_values(Handle& h) {
   hipStream_t s = h.stream;
   h.A.nslices = 0;  // the sliced-ELL copy is rebuilt with the next hierarchy
   const int npe = h.dim + 1;
   const int stride = npe == 4 ? EvLayout<4>::kStride : EvLayout<3>::kStride;
   h.asm_ev.reserve((size_t)h.ne * stride);
-  if (h.asm_maxlen <= 0) {
-    DevBuf<int> mx(1);
-    dev_fill_i32(mx.get(), 0, 1, s);
-    max_row_len_kernel<<<(unsigned)std::min<int64_t>(std::max<int64_t>((h.A.n + 255) / 256, 1), 1024), 256, 0, s>>>(
-        h.A.n, h.A.rp.get(), mx.get());
-    HIP_LAUNCH_CHECK();
-    HIP_CHECK(hipMemcpyAsync(&h.asm_maxlen, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-  }
+  if (!h.asm_ready) build_assembly_slices(h);
   const int sm = (h.flags & 1) ? 1 : 0;
   const int4* conn = (const int4*)h.conn.get();
   double* ev = h.asm_ev.get();
@@ -730,31 +860,31 @@ void assemble_values(Handle& h) {
   // algorithmic bytes: connectivity, coefficient, coordinates, incidence, CSR columns and values
   ProfScope ps(h, "assembly", (double)h.ne * (16 + 8) + (double)h.nv * 32 + (double)h.A.nnz * 12 +
                                   (double)h.ne * npe * 4);
-  const unsigned ge = grid_for(h.ne, 256), gr = grid_for(h.nv, 256);
+  const int64_t nsl = (h.nv + 63) / 64;
+  const unsigned ge = grid_for(h.ne, 256), gs = (unsigned)((nsl + 3) / 4);
+  const size_t lds = sizeof(double) * 4 * (size_t)h.asm_span;
   if (npe == 4) {
     element_values_kernel<4><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
                                                 h.kappa.get(), ev);
     HIP_LAUNCH_CHECK();
-    assemble_rows_reg_kernel<4, 16><<<gr, 256, 0, s>>>(h.nv, 0, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
-                                                       val);
-    if (h.asm_maxlen > 16)
-      assemble_rows_reg_kernel<4, 32><<<gr, 256, 0, s>>>(h.nv, 17, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
-                                                         val);
-    if (h.asm_maxlen > 32)
-      assemble_rows_long_kernel<4><<<grid_for(h.nv, 64), 64, 0, s>>>(h.nv, 33, h.v2e_ptr.get(), h.v2e.get(), conn,
-                                                                     ev, rp, ci, val);
+    assemble_slices_kernel<4><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(), h.asm_inc.get(), ev,
+                                                   rp, val, h.asm_span);
+    HIP_LAUNCH_CHECK();
+    if (h.asm_nlong)
+      assemble_rows_long_kernel<4><<<(unsigned)h.asm_nlong, 64, 0, s>>>(h.nv, h.asm_nlong, h.asm_long.get(),
+                                                                         h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp,
+                                                                         ci, val);
   } else {
     element_values_kernel<3><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
                                                 h.kappa.get(), ev);
     HIP_LAUNCH_CHECK();
-    assemble_rows_reg_kernel<3, 16><<<gr, 256, 0, s>>>(h.nv, 0, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
-                                                       val);
-    if (h.asm_maxlen > 16)
-      assemble_rows_reg_kernel<3, 32><<<gr, 256, 0, s>>>(h.nv, 17, h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp, ci,
-                                                         val);
-    if (h.asm_maxlen > 32)
-      assemble_rows_long_kernel<3><<<grid_for(h.nv, 64), 64, 0, s>>>(h.nv, 33, h.v2e_ptr.get(), h.v2e.get(), conn,
-                                                                     ev, rp, ci, val);
+    assemble_slices_kernel<3><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(), h.asm_inc.get(), ev,
+                                                   rp, val, h.asm_span);
+    HIP_LAUNCH_CHECK();
+    if (h.asm_nlong)
+      assemble_rows_long_kernel<3><<<(unsigned)h.asm_nlong, 64, 0, s>>>(h.nv, h.asm_nlong, h.asm_long.get(),
+                                                                         h.v2e_ptr.get(), h.v2e.get(), conn, ev, rp,
+                                                                         ci, val);
   }
   HIP_LAUNCH_CHECK();
 }

@@ -308,7 +308,8 @@ int fem_set_params(fem_handle* hh, const fem_params* p) {
     if (p->maxIters < 0 || p->topSize < 1 || p->maxLevels < 1 || p->preInnerIters < 0 || p->postInnerIters < 0 ||
         !(p->tolerance >= 0) || p->precision < 0 || p->precision > 2 || p->smootherType < 0 ||
         p->smootherType > 1 || p->aggregationRule < 0 || p->aggregationRule > 1 || !(p->strengthThreshold >= 0) ||
-        p->strengthThreshold >= 1.0 || p->launchMode < 0 || p->launchMode > 1)
+        p->strengthThreshold >= 1.0 || p->launchMode < 0 || p->launchMode > 1 || p->cycleType < 0 ||
+        p->cycleType > 3)
       fail(FEM_ERR_ARG, "invalid solver parameter");
     if (p->aggregatorType != 0) fail(FEM_ERR_ARG, "only aggregatorType 0 (randomized MIS) is implemented");
     if (params_shape_changed(h.prm, *p)) h.H.built = false;

@@ -145,9 +145,18 @@ struct Handle {
 
   // element-centric assembly: per-element records (off-diagonal stiffness
   // entries and the mass value, 8 doubles per tet / 4 per triangle), kept
-  // for re-assembly; largest row length of the pattern (0: not yet known)
+  // for re-assembly; the incidences of each 64-row slice, lane-interleaved,
+  // with the vertices' positions in the row (asm_inc, slice offsets in
+  // 64-entry units in asm_sptr); the largest LDS span of a slice; the slices
+  // that sum in place in the CSR values instead (asm_long)
   DevBuf<double> asm_ev;
-  int asm_maxlen = 0;
+  DevBuf<int2> asm_inc;
+  DevBuf<int32_t> asm_sptr;
+  DevBuf<int32_t> asm_long;
+  DevBuf<int32_t> asm_islong;  // per slice: 1 when it sums in place
+  int64_t asm_nlong = 0;
+  int asm_span = 0;
+  bool asm_ready = false;
 
   // level-0 operator (FP64, internal numbering)
   Csr A;

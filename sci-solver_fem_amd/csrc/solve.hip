@@ -797,7 +797,7 @@ __global__ void __launch_bounds__(kNT) pcg_dot_kernel(int64_t n, int k, const do
   block_col_partials<KP>(part, k, s_buf, partials);
 }
 
-enum FinishMode { F_STORE = 0, F_ALPHA = 1, F_RR = 2, F_BETA = 3 };
+enum FinishMode { F_STORE = 0, F_ALPHA = 1, F_RR = 2, F_BETA = 3, F_FLEX = 4 };
 
 // one block: column totals of nb partials, then the scalar update
 // 1024 threads so that ~16k partials take a few rounds of independent loads.
@@ -838,6 +838,13 @@ __global__ void __launch_bounds__(kFinNT) pcg_finish_kernel(int nb, int k, const
           rec[c] = act ? 1.0 : 0.0;
           rec[64 + c] = v;
         }
+      } else if (mode == F_FLEX) {
+        // v = z.q with q = A p of the last step: z.r_old = z.r + alpha z.q, so
+        // the flexible (Polak-Ribiere) beta = (z.r - z.r_old) / rz_old =
+        // -alpha z.q / rz_old; z.r was stored in S_RZN
+        const double rz = scal[S_RZ * k + c];
+        scal[S_BETA * k + c] = (scal[S_ACT * k + c] != 0.0 && rz != 0.0) ? -scal[S_ALPHA * k + c] * v / rz : 0.0;
+        scal[S_RZ * k + c] = scal[S_RZN * k + c];
       } else {
         double rz = scal[S_RZ * k + c];
         scal[S_BETA * k + c] = (scal[S_ACT * k + c] != 0.0 && rz != 0.0) ? v / rz : 0.0;
@@ -1128,35 +1135,65 @@ bool restrict_first(Handle& h, int l, const Csr& R, const T* r, int k) {
   return true;
 }
 
-// V-cycle below level l for the right-hand side b (level l vectors, k columns);
-// returns the buffer holding the correction (nullptr when io->z64 received it).
-// first_done: restrict_first took the first pre-smoothing step.
+// Cycle types (cycleType): 0 V; 1 W, whose coarse correction is two W-cycles
+// of the next level, the second from the first's result (cycles/wcycle.h);
+// 2 F, whose coarse correction is an F-cycle of the next level followed by a
+// V-cycle from its result (the textbook F-cycle; cycles/fcycle.h writes a
+// W-cycle first).  3 (K) is a V-cycle: the reference dispatches K_CYCLE as
+// the outer CG_Flex_Cycle itself (cycles/cycle.h:34-36, amg.cu:160-165).
+constexpr int kCycleV = 0, kCycleW = 1, kCycleF = 2;
+
+int cycle_type(const Handle& h) {
+  const int c = h.prm.cycleType;
+  return (c == kCycleW || c == kCycleF) ? c : kCycleV;
+}
+
 template <typename T>
-T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr, bool first_done = false) {
+T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io = nullptr, bool first_done = false, int type = kCycleV,
+          T* x0 = nullptr);
+
+// Coarse correction of level l, the right-hand side already restricted into
+// level l+1's b: one cycle of level l+1 (V), or two (W: W then W; F: F then
+// V).  Returns level l+1's buffer holding the correction.
+template <typename T>
+T* coarse_correction(Handle& h, int l, int k, bool next_first, int type) {
+  Level& N = *h.H.levels[l + 1];
+  T* xc = vcycle<T>(h, l + 1, LV<T>::b(N), k, nullptr, next_first, type);
+  if (type != kCycleV && l + 1 != h.H.coarse)
+    xc = vcycle<T>(h, l + 1, LV<T>::b(N), k, nullptr, false, type == kCycleW ? kCycleW : kCycleV, xc);
+  return xc;
+}
+
+// One cycle of the given type on level l for the right-hand side b (level l
+// vectors, k columns), from zero or (x0, one of level l's x / t buffers) from
+// x0; returns the buffer holding the result (nullptr when io->z64 received
+// it).  first_done: restrict_first took the first pre-smoothing step.
+template <typename T>
+T* vcycle(Handle& h, int l, const T* b, int k, Level0Io* io, bool first_done, int type, T* x0) {
   Hierarchy& H = h.H;
   Level& L = *H.levels[l];
   hipStream_t s = h.stream;
   const Csr& A = level_A(h, l);
-  if (l == H.coarse) {
+  if (l == H.coarse) {  // the direct solve does not depend on a start
     coarse_kernel<T><<<(unsigned)H.nc, 64, 0, s>>>(H.nc, k, H.cinv64.get(), b, LV<T>::x(L));
     HIP_LAUNCH_CHECK();
     return LV<T>::x(L);
   }
-  T* x = LV<T>::x(L);
-  T* t = LV<T>::t(L);
+  if (x0 && (io || first_done)) fail(FEM_ERR_STATE, "a cycle from a given start takes no fused ends");
+  T* x = x0 ? x0 : LV<T>::x(L);
+  T* t = (x == LV<T>::x(L)) ? LV<T>::t(L) : LV<T>::x(L);
   T* r = LV<T>::r(L);
   T* d = LV<T>::d(L);
   const int nu_pre = h.prm.preInnerIters;
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
-  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, true, k, io, first_done);
+  T* cur = smooth<T>(h, l, A, b, x, t, d, nu_pre, x0 == nullptr, k, x0 ? nullptr : io, first_done);
   T* oth = (cur == x) ? t : x;
   {
     ProfScope ps(h, (h.profiling && l == 0) ? "resid_l0" : "", matrix_bytes<T>(A, k) + 3.0 * A.n * k * sizeof(T));
     spmv_op<T, kResid>(A, k, cur, r, b, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   }
-  Level& N = *H.levels[l + 1];
   const bool next_first = restrict_first<T>(h, l, L.R, r, k);
-  T* xc = vcycle<T>(h, l + 1, LV<T>::b(N), k, nullptr, next_first);
+  T* xc = coarse_correction<T>(h, l, k, next_first, type);
   spmv_op<T, kAdd>(L.P, k, xc, cur, nullptr, (const T*)nullptr, nullptr, nullptr, T(0), T(0), s);
   cur = smooth<T>(h, l, A, b, cur, oth, d, nu_post, false, k, io);
   return cur;
@@ -1229,9 +1266,8 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
   sell_half_kernel<kResid><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, rr, nullptr,
                                               nullptr, 0.0f, 0.0f);
   HIP_LAUNCH_CHECK();
-  Level& N = *H.levels[1];
   const bool next_first = restrict_first<float>(h, 0, H.rh, rr, 1);
-  float* xc = vcycle<float>(h, 1, N.b32.get(), 1, nullptr, next_first);
+  float* xc = coarse_correction<float>(h, 0, 1, next_first, cycle_type(h));
   spmv_op<float, kAdd>(H.ph, 1, xc, cur, nullptr, (const float*)nullptr, nullptr, nullptr, 0.0f, 0.0f, s);
   // post-smoothing; the last step writes z and the r.z partials
   ChebSeq cp(S);
@@ -1313,8 +1349,9 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
   Level& L0 = *h.H.levels[0];
   const int64_t m = h.A.n * k;
   PrecOut o;
+  const int type = cycle_type(h);
   if (h.H.precision == 0) {
-    o.z64 = vcycle<double>(h, 0, r, k);
+    o.z64 = vcycle<double>(h, 0, r, k, nullptr, false, type);
     return o;
   }
   if (fused_level0(h, k, partials != nullptr)) {
@@ -1326,7 +1363,7 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
       io.b32 = L0.b32.get();
       io.z64 = z;
       io.partials = partials;
-      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io, first_ready);
+      float* res = vcycle<float>(h, 0, L0.b32.get(), 1, &io, first_ready, type);
       if (res || io.nb == 0) fail(FEM_ERR_STATE, "fused level-0 smoother did not write its output");
       o.nb = io.nb;
     }
@@ -1342,7 +1379,8 @@ PrecOut precondition_pcg(Handle& h, const double* r, double* z, int k, double* p
   Level0Io io;
   io.r64 = r;
   io.partials = partials;
-  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k, (partials && k > 1) ? &io : nullptr, r32_ready && first_ready);
+  o.z32 = vcycle<float>(h, 0, L0.b32.get(), k, (partials && k > 1) ? &io : nullptr, r32_ready && first_ready,
+                        type);
   o.nb = io.nb;
   return o;
 }
@@ -1863,8 +1901,15 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
       // the first iteration starts from p = z; every later one is the fixed
       // sequence below (captured once, then replayed), which follows an update
       // (so fl32(r) is in place)
+      // an F-cycle is not a symmetric preconditioner: its beta is the flexible one
+      const bool flexible = cycle_type(h) == kCycleF;
       auto iteration = [&]() {
-        const PrecOut o = precondition_dot(F_BETA, 0, true);  // z, beta, rz
+        // z, beta, rz
+        const PrecOut o = flexible ? precondition_dot(F_STORE, S_RZN, true) : precondition_dot(F_BETA, 0, true);
+        if (flexible) {
+          if (o.z32) dot(h, k, h.kq.get(), o.z32, F_FLEX, 0);
+          else dot(h, k, h.kq.get(), o.z64, F_FLEX, 0);
+        }
         if (o.z32)
           pcg_p_kernel<float><<<grid_for(m, 256), 256, 0, s>>>(m, k, o.z32, h.kp.get(), h.scal.get());
         else

@@ -11,6 +11,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from conftest import report
 from vcycle_host import HostVCycle, cheb_coefficients
@@ -209,4 +210,60 @@ def test_pcg_first_step_fused_into_update(fem, gpu, meshgen):
         err = np.linalg.norm(x_gpu - x) / np.linalg.norm(x)
         report(f"      smoother={smoother}: |x_gpu - x_host| / |x_host| = {err:.2e}")
         assert err <= 1e-9
+    s.close()
+
+
+@pytest.mark.parametrize("ctype", [1, 2])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_w_and_f_cycles(fem, gpu, meshgen, ctype, precision):
+    """cycleType 1 (W) and 2 (F): the preconditioner against the host
+    statement of the same cycle (vcycle_host.HostVCycle.cycle).
+    FP64 hierarchy: to 1e-10, and the V-cycle differs from it by more than
+    100x that (a cycle type that were ignored would give the V-cycle's z).
+    FP32 hierarchy: 20 (e_in + e_h) + 1e-6, with e_in what rounding the input
+    alone and e_h what rounding the stored operators, prolongators and D^-1
+    alone to FP32 change in the host cycle: the second coarse cycle works on
+    the residual of a nearly solved coarse problem, where the FP32 operator's
+    rounding (relative to |A||x|, not to |A x|) is of the order of the W-V
+    difference itself.  Then PCG with the cycle (F: flexible beta) reaches the
+    direct solution, in no more iterations than with the V-cycle."""
+    report(f"[gpu] cycleType={ctype} precision={precision}")
+    s = _solver(fem, meshgen, 25, 2.0, precision, 1, topSize=32, cycleType=ctype)
+    host = HostVCycle(s)
+    assert s.num_levels() >= 3  # level 1 is then cycled twice per coarse correction of level 0
+    perm = s.permutation()
+    n = perm.size
+    rng = np.random.default_rng(7 + ctype)
+    r_user = rng.normal(size=n)
+    z_user = s.apply_preconditioner(r_user)
+    r_int = r_user[perm]
+    z_int = host.cycle(0, r_int, ctype)
+    z_ref = np.empty(n)
+    z_ref[perm] = z_int
+    nz = np.linalg.norm(z_int)
+    err = np.linalg.norm(z_user - z_ref) / nz
+    dv = np.linalg.norm(host.vcycle(0, r_int) - z_int) / nz
+    if precision == 0:
+        tol = 1e-10
+        report(f"      rel error vs host cycle {err:.2e} (bound {tol:.1e}); V-cycle differs by {dv:.2e}")
+        assert err <= tol and dv > 100 * tol
+    else:
+        e_in = np.linalg.norm(host.cycle(0, r_int.astype(np.float32).astype(np.float64), ctype) - z_int) / nz
+        e_h = np.linalg.norm(host.rounded_to_fp32().cycle(0, r_int, ctype) - z_int) / nz
+        tol = 20 * (e_in + e_h) + 1e-6
+        report(f"      rel error vs host cycle {err:.2e} (e_in {e_in:.1e}, e_h {e_h:.1e}, bound {tol:.1e}); "
+               f"V-cycle differs by {dv:.2e}")
+        assert err <= tol
+    # PCG with this cycle against the direct solve
+    A = s.get_matrix_scipy().tocsc()
+    b = rng.normal(size=n)
+    s.set_params(tolerance=1e-10)
+    x, info = s.solve(b)
+    xd = spla.spsolve(A, b)
+    e = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    s.set_params(cycleType=0)
+    _, info_v = s.solve(b)
+    report(f"      PCG: {info['iterations']} iterations (V-cycle: {info_v['iterations']}), rel error {e:.1e}")
+    assert info["status"] == 0 and e <= 1e-8
+    assert info["iterations"] <= info_v["iterations"]
     s.close()

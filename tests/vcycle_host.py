@@ -15,8 +15,16 @@ sigma = theta / delta, rho_0 = 1 / sigma; step 0: d = D^-1 (b - A x) / theta
 (from x = 0: x = d = D^-1 b / theta); step s > 0: rho_s = 1 / (2 sigma - rho_{s-1}),
 d = rho_s rho_{s-1} d + (2 rho_s / delta) D^-1 (b - A x); every step x = x + d.
 Jacobi: x = x + omega D^-1 (b - A x) (from x = 0: x = omega D^-1 b).
+
+W- and F-cycles (cycleType 1, 2) differ only in the coarse correction:
+    W: x_c = W(l + 1, b_c), then x_c = W(l + 1, b_c) from x_c
+    F: x_c = F(l + 1, b_c), then x_c = V(l + 1, b_c) from x_c
+(the coarsest level is solved once), where a cycle from a start x0 smooths
+from x0 (Chebyshev step 0 from d = 0) instead of from zero.
 """
 from __future__ import annotations
+
+import copy
 
 import numpy as np
 import scipy.sparse as sp
@@ -96,3 +104,26 @@ class HostVCycle:
         xc = self.vcycle(l + 1, P.T @ r)
         x = x + P @ xc
         return self.smooth(l, b, x, self.nu_post, False)
+
+    def cycle(self, l, b, ctype=0, x0=None):
+        """One cycle of type ctype (0 V, 1 W, 2 F) from zero or from x0."""
+        if l == self.coarse:
+            return self.Cinv @ b
+        A, P = self.A[l], self.P[l]
+        x = self.smooth(l, b, x0, self.nu_pre, x0 is None)
+        bc = P.T @ (b - A @ x)
+        xc = self.cycle(l + 1, bc, ctype)
+        if ctype in (1, 2) and l + 1 != self.coarse:
+            xc = self.cycle(l + 1, bc, 1 if ctype == 1 else 0, xc)
+        x = x + P @ xc
+        return self.smooth(l, b, x, self.nu_post, False)
+
+    def rounded_to_fp32(self):
+        """The same cycle with the level operators, prolongators and D^-1
+        rounded to FP32 (FP64 arithmetic): what rounding the FP32 hierarchy's
+        stored values alone changes."""
+        h = copy.copy(self)
+        h.A = [A.astype(np.float32).astype(np.float64) for A in self.A]
+        h.P = [P.astype(np.float32).astype(np.float64) for P in self.P]
+        h.dinv = [d.astype(np.float32).astype(np.float64) for d in self.dinv]
+        return h
