@@ -10,7 +10,8 @@
  *
  * Threading: calls on one handle must be serialised by the caller; distinct
  * handles are independent.  All device work of a handle is queued on one HIP
- * stream (fem_set_stream), and every call returns only after that work is done.
+ * stream (fem_set_stream), and every call returns only after that work is done
+ * (except fem_apply_preconditioner_noalloc, which only enqueues it).
  */
 #ifndef FEM_MI355X_H
 #define FEM_MI355X_H
@@ -195,6 +196,21 @@ int fem_apply_operator(fem_handle* h, const double* x, double* y, int nrhs, int 
 
 /* z = M r: one application of the V-cycle preconditioner (user numbering). */
 int fem_apply_preconditioner(fem_handle* h, const double* r, double* z, int on_device);
+
+/* z = M r like fem_apply_preconditioner(h, r, z, 1) (bitwise the same z),
+ * allocating no device memory: no counterpart in the reference; for callers
+ * that cannot allocate (code under HIP graph capture, serving loops, a
+ * caller's own Krylov iteration).  r, z: device pointers, nv doubles each,
+ * user numbering.  The work (r and z in internal numbering, r.z partials)
+ * lives in one static device array per GPU (about 32 MiB, held until the
+ * process ends), so nv must not exceed fem_noalloc_max_rows() (2^21);
+ * larger operators return FEM_ERR_CAPACITY.  Needs fem_setup (hierarchy and
+ * work vectors) beforehand, the handle's device current and profiling off
+ * (else FEM_ERR_STATE).  Enqueues on the handle's stream and returns without
+ * synchronizing.  Every call on one GPU shares the array: calls that overlap
+ * on one GPU (other streams, other handles, other threads) race. */
+int fem_apply_preconditioner_noalloc(fem_handle* h, const double* r, double* z);
+int64_t fem_noalloc_max_rows(void);
 
 /* ---- hierarchy inspection (tests and diagnostics) ---------------------------*/
 int fem_get_num_levels(const fem_handle* h);

@@ -98,6 +98,8 @@ SIGNATURES = {
     "fem_element_grad": (C.c_int, [_P, _P, _P, _P, C.c_int]),
     "fem_apply_operator": (C.c_int, [_P, _P, _P, C.c_int, C.c_int]),
     "fem_apply_preconditioner": (C.c_int, [_P, _P, _P, C.c_int]),
+    "fem_apply_preconditioner_noalloc": (C.c_int, [_P, _P, _P]),
+    "fem_noalloc_max_rows": (C.c_int64, []),
     "fem_apply_level": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     "fem_get_level_smoother": (C.c_int, [_P, C.c_int, _PD, _PD, _PD, _PD]),
     "fem_apply_smoother_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double, _P, _P, _P, _P]),

@@ -436,7 +436,10 @@ int fem_update_kappa(fem_handle* hh, const double* kappa) {
 int fem_setup(fem_handle* hh) {
   if (!hh) return FEM_ERR_ARG;
   return guarded(hh, [&]() {
-    ensure_setup(*H(hh));
+    Handle& h = *H(hh);
+    ensure_setup(h);
+    ensure_krylov(h, 1);  // one-column work vectors (what fem_apply_preconditioner_noalloc needs)
+    noalloc_init(h);
     return (int)FEM_OK;
   });
 }
@@ -545,6 +548,16 @@ int fem_apply_preconditioner(fem_handle* hh, const double* r, double* z, int on_
     return (int)FEM_OK;
   });
 }
+
+int fem_apply_preconditioner_noalloc(fem_handle* hh, const double* r, double* z) {
+  if (!hh || !r || !z) return FEM_ERR_ARG;
+  return guarded(hh, [&]() {
+    apply_preconditioner_noalloc(*H(hh), r, z);
+    return (int)FEM_OK;
+  });
+}
+
+int64_t fem_noalloc_max_rows(void) { return noalloc_max_rows(); }
 
 int fem_apply_level(fem_handle* hh, int lvl, int which, const double* x, double* y) {
   if (!hh || !x || !y) return FEM_ERR_ARG;

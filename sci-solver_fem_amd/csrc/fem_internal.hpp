@@ -212,6 +212,12 @@ int check_mesh_device(int64_t nconn, const int32_t* conn, int64_t nv, int64_t ne
 void assemble_values(Handle& h);            // values only, pattern fixed
 void element_gradient(Handle& h, const double* a_int, const double* u_int, double* g_user);
 void gather_user_to_int(Handle& h, const double* u, double* i, int k);   // i[r] = u[perm[r]]
+// z = M r (user numbering, device pointers) through the static device scratch
+// in solve.hip: no device allocation; needs a built hierarchy and work vectors
+void apply_preconditioner_noalloc(Handle& h, const double* r_user, double* z_user);
+int64_t noalloc_max_rows();
+// resolves the scratch address of the handle's device when it is current
+void noalloc_init(const Handle& h);
 void scatter_int_to_user(Handle& h, const double* i, double* u, int k);  // u[perm[r]] = i[r]
 
 // ---- sparse building blocks (sparse.hip) ----

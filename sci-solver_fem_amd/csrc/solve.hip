@@ -16,6 +16,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "fem_internal.hpp"
@@ -1994,6 +1997,68 @@ int pcg_solve(Handle& h, int k, fem_solve_info* info) {
     info->solve_ms = ms;
   }
   return status;
+}
+
+// ---- allocation-free preconditioner application ---------------------------------
+// Device scratch of apply_preconditioner_noalloc: r and z in internal
+// numbering (kNoAllocRows doubles each), then the r.z partials of the fused
+// level-0 smoother's last step (at most one per 64-row slice).  The array is
+// one copy per GPU, held until the process ends, and shared by every call on
+// that GPU.  Its name is a host-side stand-in in host code: kernels receive
+// the per-device address from hipGetSymbolAddress.
+constexpr int64_t kNoAllocRows = (int64_t)1 << 21;
+constexpr int64_t kNoAllocPartials = kNoAllocRows / 64 + 64;
+static __device__ __align__(256) double scratch[2 * kNoAllocRows + kNoAllocPartials];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;  // device ordinal -> address of scratch
+
+// the address of scratch on device dev, which must be the current device
+// (looked up once per device)
+double* scratch_address(int dev) {
+  int cur = -1;
+  HIP_CHECK(hipGetDevice(&cur));
+  if (cur != dev)
+    fail(FEM_ERR_STATE, "the current HIP device (" + std::to_string(cur) + ") is not the handle's device (" +
+                            std::to_string(dev) + ")");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev);
+  if (it != g_scratch_addr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    fail(FEM_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_addr[dev] = static_cast<double*>(p);
+  return static_cast<double*>(p);
+}
+}  // namespace
+
+int64_t noalloc_max_rows() { return kNoAllocRows; }
+
+void noalloc_init(const Handle& h) {
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur == h.device) (void)scratch_address(h.device);
+}
+
+void apply_preconditioner_noalloc(Handle& h, const double* r_user, double* z_user) {
+  const int64_t n = h.A.n;
+  if (n == 0) fail(FEM_ERR_EMPTY, "Error no matrix specified");
+  if (n > kNoAllocRows)
+    fail(FEM_ERR_CAPACITY, "fem_apply_preconditioner_noalloc: " + std::to_string(n) +
+                               " rows exceed its limit of " + std::to_string(kNoAllocRows) + " rows");
+  if (!h.H.built) fail(FEM_ERR_STATE, "fem_apply_preconditioner_noalloc: no hierarchy; call fem_setup first");
+  bool ready = h.kcap >= 1;
+  for (auto& Lp : h.H.levels) ready = ready && Lp->wk >= 1;
+  if (!ready) fail(FEM_ERR_STATE, "fem_apply_preconditioner_noalloc: work vectors missing; call fem_setup first");
+  if (h.profiling) fail(FEM_ERR_STATE, "fem_apply_preconditioner_noalloc: profiling is enabled");
+  double* base = scratch_address(h.device);
+  double* r = base;
+  double* z = base + kNoAllocRows;
+  double* partials = base + 2 * kNoAllocRows;
+  gather_user_to_int(h, r_user, r, 1);
+  precondition(h, r, z, 1, partials);
+  scatter_int_to_user(h, z, z_user, 1);
 }
 
 }  // namespace fem
