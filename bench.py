@@ -45,12 +45,13 @@ def load_package():
 
 
 def solver_params(precision: int):
-    # tuned V-cycle (sweeps over strength threshold 0-0.03 and Chebyshev degree
-    # 2-5 at 161^3): Chebyshev(3) pre/post on D^-1 A, aggregation on the
-    # strength graph at theta = 0.01, SA-AMG down to <= 256 rows
+    # tuned V-cycle (sweeps at 161^3 over strength threshold 0-0.03, Chebyshev
+    # degree 2-5, prolongator weight 0.3-1.1, cycle type): Chebyshev(3)
+    # pre/post on D^-1 A, aggregation on the strength graph at theta = 0.01,
+    # prolongator smoothing weight 0.5, SA-AMG down to <= 256 rows
     return dict(solverType=1, tolerance=1e-8, maxIters=500, precision=precision, smootherType=1,
                 preInnerIters=3, postInnerIters=3, postRelaxes=1, topSize=256, aggregationRule=1,
-                strengthThreshold=0.01)
+                strengthThreshold=0.01, proOmega=0.5)
 
 
 def time_solves(torch, s, b, steps, warmup, profile_family=None):
@@ -98,7 +99,8 @@ def cpu_baseline(fem, size: int):
                       f"per level, FP32 hierarchy) + FP64 PCG to 1e-8 on {res['threads']} threads: "
                       f"{res['iterations']} iterations, setup {res['setup_s']:.2f} s, solve {res['solve_s']:.2f} s",
             "iterations": res["iterations"], "setup_s": res["setup_s"], "solve_s": res["solve_s"],
-            "converged": res["converged"], "rel_residual": res["rel_residual"]}
+            "converged": res["converged"], "rel_residual": res["rel_residual"], "cpu_model": cpu_model(),
+            "nproc": os.cpu_count()}
 
 
 FP64_PEAK_TFLOPS = 78.6
@@ -365,6 +367,55 @@ def run_configs(a):
     log(f"[configs] wrote {a.out}")
 
 
+def trace_solves(a):
+    """One precision of the workload: setup, one solve, a 0.5 s pause (the
+    marker iteration_budget finds in the trace), then 3 solves."""
+    import torch
+    fem = load_package()
+    mg = fem.meshgen
+    xyz, tets = mg.cube_mesh(a.size, seed=0)
+    kappa = mg.lognormal_kappa(0, tets.shape[0], a.sigma)
+    b = torch.from_numpy(np.ascontiguousarray(mg.gaussian_rhs(0, xyz.shape[0]))).to("cuda:0")
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(**solver_params(a.trace_solves))
+    s.setup()
+    s.solve(b)
+    torch.cuda.synchronize()
+    time.sleep(0.5)
+    for _ in range(3):
+        _, info = s.solve(b)
+    torch.cuda.synchronize()
+    log(f"[trace] precision={a.trace_solves} iterations={info['iterations']} res={info['rel_residual']:.2e}")
+
+
+def iteration_budget(path, iters, nsolve=3):
+    """Kernel time per PCG iteration, by kernel and grid, over the launches
+    after the longest gap of a kernel trace (the pause of trace_solves)."""
+    import collections
+    import csv
+    ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], int(r["Grid_Size_X"]))
+                for r in csv.DictReader(open(path)))
+    gap, i0 = max((ev[i + 1][0] - ev[i][1], i) for i in range(len(ev) - 1))
+    tail = ev[i0 + 1:]
+    wall = (tail[-1][1] - tail[0][0]) / 1e3
+    busy = sum(e - s for s, e, _, _ in tail) / 1e3
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for s, e, n, g in tail:
+        name = n.replace("void ", "").replace("fem::(anonymous namespace)::", "").replace("fem::", "").split("(")[0]
+        agg[(name, g)][0] += 1
+        agg[(name, g)][1] += (e - s) / 1e3
+    out = [f"{nsolve} solves of {iters} iterations after a {gap / 1e9:.2f} s pause: wall {wall / 1e3:.2f} ms, "
+           f"kernels busy {busy / 1e3:.2f} ms; per iteration wall {wall / nsolve / iters:.1f} us, "
+           f"kernels {busy / nsolve / iters:.1f} us", "",
+           "| kernel | grid threads | launches / iteration | us / launch | us / iteration |",
+           "|---|---:|---:|---:|---:|"]
+    for (name, g), (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        per = t / nsolve / iters
+        if per >= 0.5:
+            out.append(f"| `{name}` | {g} | {c / nsolve / iters:.2f} | {t / c:.1f} | {per:.1f} |")
+    return "\n".join(out)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         # bench.py --pmc-summary FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv COPY_MIB ALGO_BYTES "command"
@@ -390,7 +441,19 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="run every workload (C1-C5) once with per-config rows instead of the headline line")
     ap.add_argument("--out", default="gpurun_out/configs.json", help="--configs: JSON output path")
+    ap.add_argument("--trace-solves", type=int, default=-1, metavar="PRECISION",
+                    help="only: build the workload in one precision, one solve, a 0.5 s pause, then 3 solves "
+                         "(for kernel traces; --iteration-budget reads the trace)")
+    ap.add_argument("--iteration-budget", nargs=2, metavar=("TRACE_CSV", "ITERATIONS"),
+                    help="only: per-iteration kernel budget of the 3 solves after the pause in a rocprofv3 "
+                         "kernel trace of --trace-solves (markdown)")
     a = ap.parse_args()
+    if a.iteration_budget:
+        print(iteration_budget(a.iteration_budget[0], int(a.iteration_budget[1])))
+        return
+    if a.trace_solves >= 0:
+        trace_solves(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
               file=sys.stderr)

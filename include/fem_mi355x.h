@@ -57,7 +57,11 @@ typedef struct fem_params {
   int randMisParameters;  /* randMisParameters_  90102 digits: min aggregate
                              size (x/10000%10), coarse depth, fine MIS depth */
   int partitionMaxSize;   /* partitionMaxSize_   512 accepted, no effect      */
-  int aggregatorType;     /* aggregatorType_     0  (only 0 is implemented)   */
+  int aggregatorType;     /* aggregatorType_     0  randomized MIS-2; 3 AggMIS:
+                             MIS, aggregate-to-nearest, merge-split to sizes
+                             [p/1000%1000, p%1000] of randMisParameters p
+                             (MIS depth p/10^7%10, 0 = 2); 1, 2, 4, 5: not
+                             built (FEM_ERR_ARG)                              */
   int convergeType;       /* convergeType_       0  accepted, no effect       */
   double tolerance;       /* tolerance_          1e-6 relative residual       */
   int cycleType;          /* cycleType_          0  V; 1 W (coarse correction:
@@ -261,6 +265,15 @@ int fem_get_permutation(const fem_handle* h, int32_t* perm);
 int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj,
                          int depth, int min_size, int rule, unsigned long long seed,
                          int32_t* aggregate, int32_t* n_agg);
+
+/* Condition an aggregation (ids 0..n_agg-1, all in use) of a host CSR graph
+ * (no diagonal) to sizes [min_size, max_size] with the merge-split
+ * conditioner of aggregatorType 3 (MergeSplitConditionerGPU::Condition,
+ * src/core/aggmis/cuda/AggMIS_MergeSplitConditioner.cu:1625-1715, as NewMIS
+ * calls it, src/core/cuda/ComputePermutationMethods.cu:403-407).  Host only;
+ * rewrites aggregate and returns the new count in *n_agg_out. */
+int fem_condition_aggregates(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* aggregate,
+                             int64_t n_agg, int min_size, int max_size, int64_t* n_agg_out);
 
 /* ---- profiling -------------------------------------------------------------*/
 /* Time each launch of the named kernel family with HIP events on the handle's

@@ -42,3 +42,11 @@ int orc_amg_pcg(int64_t n, const int64_t* rp, const int32_t* ci, const double* v
 #ifdef __cplusplus
 }
 #endif
+#ifdef __cplusplus
+extern "C" {
+#endif
+int64_t orc_condition(int64_t n, const int32_t* ap, const int32_t* aj, int32_t* agg, int64_t nagg, int min_size,
+                      int max_size);
+#ifdef __cplusplus
+}
+#endif

@@ -914,6 +914,16 @@ void build_hierarchy(Handle& h) {
   const int fine_depth = h.prm.randMisParameters % 100;
   const int min_size = (h.prm.randMisParameters / 10000) % 10;
   const double omega = h.prm.proOmega;
+  // aggregatorType 3 (AggMIS "NewMIS" fine aggregation,
+  // src/core/cuda/ComputePermutationMethods.cu:352-431): its own reading of
+  // randMisParameters -- sizes [p / 1000 % 1000, p % 1000], MIS depth
+  // p / 10^7 % 10 -- then MIS, aggregate-to-nearest and the merge-split
+  // conditioner.  Depth 0 (the default's digit) is taken as 2: the
+  // reference's depth-0 MIS marks no root and its aggregate-to-nearest loop
+  // never ends.
+  const bool aggmis = h.prm.aggregatorType == 3;
+  const int agg_lo = (h.prm.randMisParameters / 1000) % 1000, agg_hi = h.prm.randMisParameters % 1000;
+  const int agg_depth = (h.prm.randMisParameters / 10000000) % 10 ? (h.prm.randMisParameters / 10000000) % 10 : 2;
 
   // level-0 graph: pattern of A without the diagonal
   Csr G;
@@ -955,15 +965,32 @@ void build_hierarchy(Handle& h) {
     if (!last) {
       L.agg.resize(n);
       unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
-      aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
-                      &nagg, s);
+      auto aggregate = [&]() {
+        if (!aggmis) {
+          aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
+                          &nagg, s);
+          return;
+        }
+        // MIS + aggregate-to-nearest on the device (AggregateToNearest,
+        // src/core/aggmis/cuda/AggMIS_Aggregation_GPU.cu:114-158: its
+        // candidate table makes it the first-aggregated-neighbour rule, no
+        // runt removal), then the conditioner on the host
+        aggregate_graph(n, G.rp.get(), G.ci.get(), agg_depth, 0, 0, seed, L.agg.get(), &nagg, s);
+        std::vector<int32_t> grp(n + 1), gci(G.nnz), hagg(n);
+        G.rp.download(grp.data(), n + 1, s);
+        G.ci.download(gci.data(), G.nnz, s);
+        L.agg.download(hagg.data(), n, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        nagg = condition_aggregates(n, grp.data(), gci.data(), hagg, nagg, agg_lo, agg_hi);
+        L.agg.upload(hagg.data(), n, s);
+      };
+      aggregate();
       if (theta > 0.0 && 2 * nagg > n) {
         // The strength graph fell apart into small components (each left as
         // singletons after runt removal): coarsening would stall for many
         // levels.  Aggregate this level on the full pattern instead.
         offdiag_graph(A, G, s);
-        aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
-                        &nagg, s);
+        aggregate();
       }
     }
     if (last || nagg >= n || nagg == 0) {

@@ -388,6 +388,7 @@ __device__ __forceinline__ void corner_row(int code, const int4* __restrict__ co
 }
 
 constexpr int kSliceSpan = 2048;  // LDS doubles per wavefront (16 KiB)
+constexpr int kUnroll = 2;        // incidences in flight per lane in assemble_slices_kernel
 
 // per slice of 64 rows: the most incidences of a row (0 for a slice that
 // takes the in-place path) and whether the slice takes that path
@@ -500,16 +501,20 @@ __global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const 
       }
     };
     int k = 0;
-    // two incidences loaded at a time (independent load chains), added in order
-    for (; k + 1 < kn; k += 2) {
-      const int2 i0 = q[(int64_t)k << 6], i1 = q[(int64_t)(k + 1) << 6];
-      double r0[8], r1[8];
-      if (i0.x >= 0) load_record<NPE>(ev, i0.x >> 2, r0);
-      if (i1.x >= 0) load_record<NPE>(ev, i1.x >> 2, r1);
-      if (i0.x >= 0) add(i0, r0);
-      if (i1.x >= 0) add(i1, r1);
+    // kUnroll incidences loaded at a time (independent load chains), added in order
+    for (; k + kUnroll - 1 < kn; k += kUnroll) {
+      int2 ic[kUnroll];
+      double r[kUnroll][8];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) ic[u] = q[(int64_t)(k + u) << 6];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        if (ic[u].x >= 0) load_record<NPE>(ev, ic[u].x >> 2, r[u]);
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        if (ic[u].x >= 0) add(ic[u], r[u]);
     }
-    if (k < kn) {
+    for (; k < kn; ++k) {
       const int2 i0 = q[(int64_t)k << 6];
       if (i0.x >= 0) {
         double r0[8];

@@ -311,7 +311,14 @@ int fem_set_params(fem_handle* hh, const fem_params* p) {
         p->strengthThreshold >= 1.0 || p->launchMode < 0 || p->launchMode > 1 || p->cycleType < 0 ||
         p->cycleType > 3)
       fail(FEM_ERR_ARG, "invalid solver parameter");
-    if (p->aggregatorType != 0) fail(FEM_ERR_ARG, "only aggregatorType 0 (randomized MIS) is implemented");
+    if (p->aggregatorType != 0 && p->aggregatorType != 3)
+      fail(FEM_ERR_ARG, "aggregatorType: 0 (randomized MIS) and 3 (AggMIS merge-split) are implemented");
+    if (p->aggregatorType == 3) {
+      const int lo = (p->randMisParameters / 1000) % 1000, hi = p->randMisParameters % 1000;
+      if (p->randMisParameters < 0 || lo < 1 || hi < lo)
+        fail(FEM_ERR_ARG, "aggregatorType 3 reads aggregate sizes [p / 1000 % 1000, p % 1000] from "
+                          "randMisParameters; they must satisfy 1 <= min <= max");
+    }
     if (params_shape_changed(h.prm, *p)) h.H.built = false;
     if (p->device != h.prm.device) fail(FEM_ERR_ARG, "device cannot change after construction");
     h.prm = *p;
@@ -682,6 +689,29 @@ int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj, 
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipStreamDestroy(s));
     *n_agg = (int32_t)na;
+    return (int)FEM_OK;
+  });
+}
+
+int fem_condition_aggregates(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* aggregate,
+                             int64_t n_agg, int min_size, int max_size, int64_t* n_agg_out) {
+  if (n < 0 || !adj_ptr || !aggregate || !n_agg_out || n_agg < 0 || min_size < 1 || max_size < min_size)
+    return FEM_ERR_ARG;
+  return guarded(nullptr, [&]() {
+    const int64_t m = adj_ptr[n];
+    if (m > 0 && !adj) fail(FEM_ERR_ARG, "adjacency missing");
+    for (int64_t t = 0; t < m; ++t)
+      if (adj[t] < 0 || adj[t] >= n) fail(FEM_ERR_ARG, "adjacency index out of range");
+    std::vector<int32_t> agg(aggregate, aggregate + n);
+    std::vector<char> used((size_t)n_agg, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      if (agg[(size_t)i] < 0 || agg[(size_t)i] >= n_agg) fail(FEM_ERR_ARG, "aggregate id out of range");
+      used[(size_t)agg[(size_t)i]] = 1;
+    }
+    for (int64_t a = 0; a < n_agg; ++a)
+      if (!used[(size_t)a]) fail(FEM_ERR_ARG, "aggregate ids must be dense (every id in use)");
+    *n_agg_out = condition_aggregates(n, adj_ptr, adj, agg, n_agg, min_size, max_size);
+    std::copy(agg.begin(), agg.end(), aggregate);
     return (int)FEM_OK;
   });
 }

@@ -241,6 +241,10 @@ void spmv64(const Csr& A, const double* x, double* y, hipStream_t s);
 
 // ---- AMG setup (amg_setup.hip) ----
 void build_hierarchy(Handle& h);
+// AggMIS merge-split conditioning of an aggregation to sizes [min_size,
+// max_size] (host; aggmis.cpp); rewrites agg, returns the aggregate count
+int64_t condition_aggregates(int64_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& agg,
+                             int64_t nagg, int min_size, int max_size);
 void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth, int min_size,
                      int rule, unsigned long long seed, int32_t* agg_out, int64_t* n_agg,
                      hipStream_t s);

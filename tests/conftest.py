@@ -77,6 +77,8 @@ class Oracle:
         lib.orc_induced_graph.argtypes = [C.c_int64, V, V, V, C.c_int64, C.POINTER(C.POINTER(C.c_int32)),
                                           C.POINTER(C.POINTER(C.c_int32))]
         lib.orc_free.argtypes = [V]
+        lib.orc_condition.restype = C.c_int64
+        lib.orc_condition.argtypes = [C.c_int64, V, V, V, C.c_int64, C.c_int, C.c_int]
         lib.orc_amg_pcg.restype = C.c_int
         lib.orc_amg_pcg.argtypes = [C.c_int64, V, V, V, V, V, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, V]
@@ -160,6 +162,15 @@ class Oracle:
         out = np.empty(ap.size - 1, dtype=np.int32)
         na = self.lib.orc_aggregate(ap.size - 1, ap.ctypes.data, aj.ctypes.data, depth, min_size, rule, seed,
                                     out.ctypes.data)
+        return out, int(na)
+
+    def condition(self, ap, aj, agg, nagg, min_size, max_size):
+        """AggMIS merge-split conditioning (orc_condition); returns (agg, nagg)."""
+        ap = np.ascontiguousarray(ap, dtype=np.int32)
+        aj = np.ascontiguousarray(aj, dtype=np.int32)
+        out = np.array(agg, dtype=np.int32, copy=True)
+        na = self.lib.orc_condition(ap.size - 1, ap.ctypes.data, aj.ctypes.data, out.ctypes.data, nagg, min_size,
+                                    max_size)
         return out, int(na)
 
     def induced_graph(self, ap, aj, agg, nc):

@@ -54,3 +54,44 @@ def test_mis_invariants(oracle, meshgen):
         # maximality: every non-root within distance k of a root
         covered = np.asarray(Gk[:, roots].sum(axis=1)).ravel() > 0
         assert np.all(covered[mis == 0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_aggmis_hierarchy(fem, gpu, oracle, meshgen, precision):
+    """aggregatorType 3 (randMisParameters 20027036: MIS depth 2, sizes
+    [27, 36]): the level-0 aggregates of the hierarchy equal the oracle's
+    pipeline on the same graph and seed (device MIS + first-aggregated-
+    neighbour allocation without runt removal, then the merge-split
+    conditioner), at most 10 % of them lie outside [27, 36], and PCG on this
+    hierarchy reaches the direct solution."""
+    import scipy.sparse.linalg as spla
+    report(f"[gpu] aggregatorType 3 hierarchy, precision={precision}")
+    xyz, tets = meshgen.cube_mesh(21, seed=3)
+    kappa = meshgen.lognormal_kappa(3, tets.shape[0], 1.0)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-10, maxIters=300, precision=precision, smootherType=1,
+                 preInnerIters=3, postInnerIters=3, topSize=64, aggregatorType=3, randMisParameters=20027036,
+                 strengthThreshold=0.0, seed=0x5EED)
+    s.setup()
+    rp, ci, v, agg = s.level_matrix(0)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = ci != rows
+    ap = np.concatenate([[0], np.cumsum(np.add.reduceat(keep, rp[:-1]))]).astype(np.int32)
+    aj = ci[keep].astype(np.int32)
+    a0, n0 = oracle.aggregate(ap, aj, depth=2, min_size=0, rule=0, seed=0x5EED)
+    want, nw = oracle.condition(ap, aj, a0, n0, 27, 36)
+    sizes = np.bincount(agg)
+    report(f"      level 0: {n} rows -> {sizes.size} aggregates of {sizes.min()}..{sizes.max()}, "
+           f"{s.num_levels()} levels")
+    assert sizes.size == nw and np.array_equal(agg, want)
+    assert np.sum((sizes < 27) | (sizes > 36)) <= 0.1 * nw
+    b = meshgen.gaussian_rhs(3, n)
+    x, info = s.solve(b)
+    A_user = s.get_matrix_scipy()
+    xd = spla.spsolve(A_user.tocsc(), b)
+    e = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    report(f"      PCG: {info['iterations']} iterations, rel error {e:.1e}")
+    assert info["status"] == 0 and e <= 1e-8
+    s.close()
