@@ -441,6 +441,17 @@ int64_t orc_condition(int64_t n, const int32_t* ap, const int32_t* aj, int32_t* 
   }
   cycle_splits(&c, 1);
   cycle_merges(&c, 0);
+  /* drop ids without nodes (a split whose two roots coincide), keeping order */
+  {
+    int32_t* id = (int32_t*)malloc(sizeof(int32_t) * (size_t)(c.nagg + 1));
+    int32_t next = 0;
+    for (int64_t a = 0; a < c.nagg; ++a) id[a] = c.sizes[a] > 0 ? next++ : -1;
+    if (next != c.nagg) {
+      for (int64_t i = 0; i < n; ++i) agg[i] = id[agg[i]];
+      c.nagg = next;
+    }
+    free(id);
+  }
   int64_t out = c.nagg;
   free(c.sizes);
   free(c.gp);

@@ -60,6 +60,7 @@ class MergeSplit {
     }
     cycle_splits(true);
     cycle_merges(false);
+    compact();
     return nagg_;
   }
 
@@ -316,6 +317,19 @@ class MergeSplit {
     }
     agg_.swap(result);
     nagg_ = base + (int64_t)order.size();
+    rebuild();
+  }
+
+  // ids without nodes (a split whose two roots coincide) are removed, the
+  // others keep their order
+  void compact() {
+    std::vector<int32_t> id((size_t)nagg_, -1);
+    int32_t next = 0;
+    for (int64_t a = 0; a < nagg_; ++a)
+      if (size_[(size_t)a] > 0) id[(size_t)a] = next++;
+    if (next == nagg_) return;
+    for (int32_t& g : agg_) g = id[(size_t)g];
+    nagg_ = next;
     rebuild();
   }
 

@@ -890,6 +890,8 @@ void build_half_level0(Handle& h) {
                                                      H.ah_diag.get(), H.ah_s.get());
   HIP_LAUNCH_CHECK();
   build_sell(O, false, s, 0, /*with_f16=*/true, /*with_f64=*/false);
+  // y staged in LDS per block of slices, as for the FP32 fine operator
+  build_lds(O, lds_block_slices(O), s);
   O.v64.release();  // only the FP16 sliced copy is used
   // D^1/2 P (sliced ELL, FP32, rows sorted by length) and its transpose (CSR, FP32)
   copy_csr(L0.P, H.ph, s);

@@ -410,6 +410,81 @@ __global__ void __launch_bounds__(kNT) sell_half_out_kernel(int64_t n, int64_t n
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// The same two kernels on the LDS-staged copy: a block stages y at its
+// slices' distinct columns once, and every entry reads y from LDS through its
+// 16-bit index (same entries in the same order, so the same sums).
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_half_lds_kernel(int64_t n, int64_t nslices, SellView S, LdsView L,
+                                                          const uint16_t* __restrict__ sh,
+                                                          const float* __restrict__ diag, const float* __restrict__ y,
+                                                          const float* __restrict__ bh, float* __restrict__ out,
+                                                          float* __restrict__ d, float* __restrict__ yn, float c1,
+                                                          float c2) {
+  extern __shared__ unsigned char lds_raw[];
+  float* ys = reinterpret_cast<float*>(lds_raw);
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  lds_stage<float>(L, blk, y, ys);
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x >> 6; q < L.bs; q += kNT / 64) {
+    const int slice = __builtin_amdgcn_readfirstlane((int)(blk * L.bs + q));
+    if (slice >= nslices) break;
+    const float acc = lds_row_sum<float, uint16_t>(S, L, slice, lane, sh, ys);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row >= n) continue;
+    const float yi = y[row];
+    const float r = ntload(bh + row) - (ntload(diag + row) * yi + acc);
+    if (MODE == kResid) {
+      out[row] = r;
+    } else if (MODE == kJacobi) {
+      yn[row] = yi + c1 * r;
+    } else {
+      const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * r;
+      d[row] = dn;
+      yn[row] = yi + dn;
+    }
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kNT) sell_half_lds_out_kernel(int64_t n, int64_t nslices, SellView S, LdsView L,
+                                                              const uint16_t* __restrict__ sh,
+                                                              const float* __restrict__ diag,
+                                                              const float* __restrict__ sd,
+                                                              const float* __restrict__ y,
+                                                              const float* __restrict__ bh,
+                                                              const float* __restrict__ d, float c1, float c2,
+                                                              const double* __restrict__ r64, double* __restrict__ z,
+                                                              double* __restrict__ partials) {
+  extern __shared__ unsigned char lds_raw[];
+  float* ys = reinterpret_cast<float*>(lds_raw);
+  __shared__ double s_buf[kNT];
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  lds_stage<float>(L, blk, y, ys);
+  const int lane = threadIdx.x & 63;
+  double part = 0.0;
+  for (int q = threadIdx.x >> 6; q < L.bs; q += kNT / 64) {
+    const int slice = __builtin_amdgcn_readfirstlane((int)(blk * L.bs + q));
+    if (slice >= nslices) break;
+    const float acc = lds_row_sum<float, uint16_t>(S, L, slice, lane, sh, ys);
+    const int64_t row = (int64_t)slice * 64 + lane;
+    if (row >= n) continue;
+    const float yi = y[row];
+    const float r = ntload(bh + row) - (ntload(diag + row) * yi + acc);
+    float yv;
+    if (MODE == kJacobi) {
+      yv = yi + c1 * r;
+    } else {
+      const float dn = (c1 != 0.0f ? c1 * ntload(d + row) : 0.0f) + c2 * r;
+      yv = yi + dn;
+    }
+    const double zv = (double)(sd[row] * yv);
+    z[row] = zv;
+    part += r64[row] * zv;
+  }
+  const double tot = block_sum(part, s_buf);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
 template <typename Tin, typename Tout>
 __global__ void convert_kernel(int64_t m, const Tin* __restrict__ a, Tout* __restrict__ b) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1240,13 +1315,26 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
   const int nu_post = h.prm.postInnerIters * (h.prm.postRelaxes > 0 ? h.prm.postRelaxes : 1);
   const unsigned g = grid_for(O.nslices, kNT / 64);
   const SellView V = sell_view(O);
+  // LDS-staged copy when its lists exist: one block per O.lds_bs slices
+  const bool staged = O.lds_bs > 0;
+  const LdsView LV_ = staged ? lds_view(O) : LdsView{};
+  const unsigned gl = staged ? (unsigned)((O.nslices + O.lds_bs - 1) / O.lds_bs) : 0;
+  const size_t ldsb = staged ? sizeof(float) * (size_t)O.lds_maxc : 0;
   // algorithmic bytes: FP16 value + column code per off-diagonal, and y, b^, diag (+ d r/w), y_new
-  const double mbytes = 2.0 * (double)O.nnz + 2.0 * (double)O.nnz_compact + 4.0 * ((double)O.nnz - O.nnz_compact);
+  const double mbytes = staged ? 2.0 * (double)O.nnz + 2.0 * (double)O.nnz + 4.0 * (double)O.lcols.size()
+                               : 2.0 * (double)O.nnz + 2.0 * (double)O.nnz_compact +
+                                     4.0 * ((double)O.nnz - O.nnz_compact);
   const double step_bytes = mbytes + 4.0 * n * (S.type ? 6.0 : 4.0);
   const bool prof = h.profiling;
   auto step = [&](int mode, double c1, double c2) {
     ProfScope ps(h, prof ? "smooth_l0_h" : "", step_bytes);
-    if (mode == kJacobi)
+    if (staged && mode == kJacobi)
+      sell_half_lds_kernel<kJacobi><<<gl, kNT, ldsb, s>>>(n, O.nslices, V, LV_, O.sh.get(), H.ah_diag.get(), cur,
+                                                           bh, nullptr, nullptr, nxt, (float)c1, 0.0f);
+    else if (staged)
+      sell_half_lds_kernel<kCheb><<<gl, kNT, ldsb, s>>>(n, O.nslices, V, LV_, O.sh.get(), H.ah_diag.get(), cur, bh,
+                                                         nullptr, d, nxt, (float)c1, (float)c2);
+    else if (mode == kJacobi)
       sell_half_kernel<kJacobi><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, nullptr,
                                                   nullptr, nxt, (float)c1, 0.0f);
     else
@@ -1266,8 +1354,12 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
     step(S.type ? kCheb : kJacobi, c1, c2);
   }
   // residual, restriction, coarse correction, prolongation
-  sell_half_kernel<kResid><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, rr, nullptr,
-                                              nullptr, 0.0f, 0.0f);
+  if (staged)
+    sell_half_lds_kernel<kResid><<<gl, kNT, ldsb, s>>>(n, O.nslices, V, LV_, O.sh.get(), H.ah_diag.get(), cur, bh,
+                                                        rr, nullptr, nullptr, 0.0f, 0.0f);
+  else
+    sell_half_kernel<kResid><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), cur, bh, rr, nullptr,
+                                                nullptr, 0.0f, 0.0f);
   HIP_LAUNCH_CHECK();
   const bool next_first = restrict_first<float>(h, 0, H.rh, rr, 1);
   float* xc = coarse_correction<float>(h, 0, 1, next_first, cycle_type(h));
@@ -1282,6 +1374,18 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
       continue;
     }
     ProfScope ps(h, prof ? "smooth_l0_h_out" : "", step_bytes + 12.0 * n);
+    if (staged) {
+      if (S.type)
+        sell_half_lds_out_kernel<kCheb><<<gl, kNT, ldsb, s>>>(n, O.nslices, V, LV_, O.sh.get(), H.ah_diag.get(),
+                                                               H.ah_s.get(), cur, bh, d, (float)c1, (float)c2, r64,
+                                                               z64, partials);
+      else
+        sell_half_lds_out_kernel<kJacobi><<<gl, kNT, ldsb, s>>>(n, O.nslices, V, LV_, O.sh.get(), H.ah_diag.get(),
+                                                                 H.ah_s.get(), cur, bh, nullptr, (float)c1, 0.0f,
+                                                                 r64, z64, partials);
+      HIP_LAUNCH_CHECK();
+      continue;
+    }
     if (S.type)
       sell_half_out_kernel<kCheb><<<g, kNT, 0, s>>>(n, O.nslices, V, O.sh.get(), H.ah_diag.get(), H.ah_s.get(), cur,
                                                     bh, d, (float)c1, (float)c2, r64, z64, partials);
@@ -1290,7 +1394,7 @@ unsigned vcycle0_half(Handle& h, const double* r64, double* z64, double* partial
                                                       cur, bh, nullptr, (float)c1, 0.0f, r64, z64, partials);
     HIP_LAUNCH_CHECK();
   }
-  return g;
+  return staged ? gl : g;
 }
 
 // Where one application of the preconditioner left z = M r.

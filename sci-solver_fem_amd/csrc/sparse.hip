@@ -447,18 +447,30 @@ __global__ void sell_compact_nnz_kernel(int64_t n, int64_t nslices, int g, const
   out[sl] = cnt;
 }
 
+// CSR entries of a slice staged in LDS by sell_fill_lds_kernel (slices of
+// one lane per row in row order whose rows hold at most this many entries)
+constexpr int kFillSpan = 1024;
+
+// the slice's span of CSR entries when its 64 rows are consecutive
+__device__ __forceinline__ int fill_span(int64_t n, int64_t sl, const int32_t* __restrict__ rp) {
+  const int64_t r0 = sl * 64, r1 = r0 + 64 < n ? r0 + 64 : n;
+  return r0 < n ? rp[r1] - rp[r0] : 0;
+}
+
 // one thread per (slice, lane): position = slice * (64 / g) + lane / g, and
-// the lane holds entries lane % g, lane % g + g, ... of that row
+// the lane holds entries lane % g, lane % g + g, ... of that row.  With
+// skip_staged, slices that sell_fill_lds_kernel fills are left to it.
 __global__ void sell_fill_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                  const int32_t* __restrict__ ci, const double* __restrict__ v,
                                  const int32_t* __restrict__ srow, const int32_t* __restrict__ sp,
                                  const int32_t* __restrict__ swin, int32_t* __restrict__ sc,
                                  uint16_t* __restrict__ sc16, double* __restrict__ s64, float* __restrict__ s32,
-                                 uint16_t* __restrict__ sh) {
+                                 uint16_t* __restrict__ sh, int skip_staged) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslices * 64) return;
   int64_t sl = t >> 6;
   int lane = (int)(t & 63);
+  if (skip_staged && fill_span(n, sl, rp) <= kFillSpan) return;
   const int sub = lane % g;
   const int64_t pos = sl * (64 / g) + lane / g;
   int64_t row = pos < n ? sell_row(srow, pos) : pos;
@@ -492,6 +504,67 @@ __global__ void sell_fill_kernel(int64_t n, int64_t nslices, int g, const int32_
       sc16[e] = 0;
     }
     const double val = j < len ? v[base + j] : 0.0;
+    if (s64) s64[e] = val;
+    if (s32) s32[e] = (float)val;
+    if (sh) sh[e] = half_bits(val);
+  }
+}
+
+// sell_fill_kernel for one lane per row in row order, one wavefront per
+// slice: the slice's rows are consecutive in the CSR arrays, so the
+// wavefront reads their columns and values as one coalesced run into LDS;
+// each lane then writes its row's entries in sliced order (the same values
+// the generic kernel writes).  Slices spanning more than kFillSpan entries
+// are left to sell_fill_kernel.
+__global__ void __launch_bounds__(64) sell_fill_lds_kernel(int64_t n, int64_t nslices, const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci,
+                                                           const double* __restrict__ v,
+                                                           const int32_t* __restrict__ sp,
+                                                           const int32_t* __restrict__ swin,
+                                                           int32_t* __restrict__ sc, uint16_t* __restrict__ sc16,
+                                                           double* __restrict__ s64, float* __restrict__ s32,
+                                                           uint16_t* __restrict__ sh) {
+  __shared__ int32_t lci[kFillSpan];
+  __shared__ double lv[kFillSpan];
+  const int64_t sl = blockIdx.x;
+  if (sl >= nslices) return;
+  const int span = fill_span(n, sl, rp);
+  if (span > kFillSpan) return;
+  const int lane = threadIdx.x;
+  const int64_t r0 = sl * 64;
+  const int b0 = r0 < n ? rp[r0] : 0;
+  for (int t = lane; t < span; t += 64) {
+    lci[t] = ci[b0 + t];
+    lv[t] = v[b0 + t];
+  }
+  __syncthreads();
+  const int64_t row = r0 + lane;
+  int len = 0, base = 0;
+  if (row < n) {
+    base = rp[row] - b0;
+    len = rp[row + 1] - rp[row];
+  }
+  const int off = sp[sl];
+  const int w = (sp[sl + 1] - off) >> 6;
+  const int32_t* win = swin + sl * kSellWindows;
+  const bool compact = win[0] >= 0;
+  const int pad_col = len > 0 ? lci[base + len - 1] : (compact ? win[0] : 0);
+  for (int k = 0; k < w; ++k) {
+    const int64_t e = off + (int64_t)k * 64 + lane;
+    const int col = k < len ? lci[base + k] : pad_col;
+    sc[e] = col;
+    if (compact) {
+      int slot = 0;
+      for (int i = 0; i < kSellWindows; ++i)
+        if (win[i] == ((col >> kSellWinBits) << kSellWinBits)) {
+          slot = i;
+          break;
+        }
+      sc16[e] = (uint16_t)((slot << kSellWinBits) | (col - win[slot]));
+    } else {
+      sc16[e] = 0;
+    }
+    const double val = k < len ? lv[base + k] : 0.0;
     if (s64) s64[e] = val;
     if (s32) s32[e] = (float)val;
     if (sh) sh[e] = half_bits(val);
@@ -959,11 +1032,21 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
                                                               snnz.get());
     HIP_LAUNCH_CHECK();
     A.nnz_compact = dev_exclusive_scan_i64(snnz.get(), sscan.get(), ns, s);
+    // one lane per row in row order: slices of short rows fill through LDS
+    const bool staged = lanes == 1 && srow == nullptr;
+    if (staged) {
+      sell_fill_lds_kernel<<<(unsigned)ns, 64, 0, s>>>(A.n, ns, A.rp.get(), A.ci.get(), A.v64.get(), A.sp.get(),
+                                                       A.swin.get(), A.sc.get(), A.sc16.get(),
+                                                       with_f64 ? A.s64.get() : nullptr,
+                                                       with_f32 ? A.s32.get() : nullptr,
+                                                       with_f16 ? A.sh.get() : nullptr);
+      HIP_LAUNCH_CHECK();
+    }
     sell_fill_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), A.v64.get(),
                                                             srow, A.sp.get(), A.swin.get(), A.sc.get(), A.sc16.get(),
                                                             with_f64 ? A.s64.get() : nullptr,
                                                             with_f32 ? A.s32.get() : nullptr,
-                                                            with_f16 ? A.sh.get() : nullptr);
+                                                            with_f16 ? A.sh.get() : nullptr, staged ? 1 : 0);
     HIP_LAUNCH_CHECK();
   }
   HIP_CHECK(hipStreamSynchronize(s));
