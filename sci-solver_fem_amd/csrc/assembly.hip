@@ -272,8 +272,8 @@ __device__ __forceinline__ void element_geometry(const double4* __restrict__ X, 
 // The element matrix is then K_ab = S_ab + m (a != b) and, because the
 // stiffness rows of a P1 element sum to zero, K_aa = -sum_{b != a} S_ab + 2 m.
 //
-// Pass 2 (assemble_slices_kernel): a wavefront owns a slice of 64 consecutive
-// rows; each lane walks the elements incident to its row in ascending element
+// Pass 2 (assemble_slices_kernel): a block owns a slice of 64 consecutive
+// rows; each row walks the elements incident to it in ascending element
 // order and adds the row of each element matrix into the row's entries.
 // Every entry is a sum over its elements in ascending element order, and
 // entry (i, j) and entry (j, i) add the same stored S_ab + m, so A is
@@ -283,7 +283,7 @@ __device__ __forceinline__ void element_geometry(const double4* __restrict__ X, 
 // positions of the element's vertices in the row (one byte each), so no
 // column search runs at assembly time.  The slice's rows are contiguous in
 // the CSR value array: they are summed in LDS at their CSR offsets and
-// written back by the wavefront as one coalesced run.  Slices whose rows
+// written back by the block as one coalesced run.  Slices whose rows
 // span more than kSliceSpan entries, or hold a row longer than 255 entries,
 // sum in place in the CSR values with a column search (any row length).
 
@@ -306,40 +306,56 @@ struct EvLayout<3> {
   __device__ static int slot(int a, int b) { return a + b - 1; }
 };
 
+// One thread per element; the block's records (consecutive elements) are
+// staged in LDS and written back as one contiguous run, each store
+// instruction covering 64 consecutive 16-byte pieces (a thread's own 64-byte
+// record written directly would be four strided 16-byte pieces per lane).
 template <int NPE>
 __global__ void __launch_bounds__(256) element_values_kernel(int64_t ne, int signed_mass, double mass_coeff,
                                                              const int4* __restrict__ conn,
                                                              const double4* __restrict__ X,
                                                              const double* __restrict__ kappa,
                                                              double* __restrict__ ev) {
+  constexpr int S = EvLayout<NPE>::kStride;
+  constexpr int P = S + 1;  // padded LDS stride (doubles): conflict-free record writes
+  __shared__ double rec[256 * P];
   // XCD-contiguous block order: neighbouring elements share vertices, and
   // each XCD's L2 then reads a contiguous range of the coordinates
-  const int64_t e = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int4 c = conn[e];
-  Geo G;
-  element_geometry(X, c, NPE - 1, G);
-  const double kv = kappa[e] * G.meas;
-  const double m = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * (NPE == 4 ? 1.0 / 20.0 : 1.0 / 12.0);
-  double* o = ev + e * EvLayout<NPE>::kStride;
-  if (NPE == 4) {
-    double v[8];
-    int t = 0;
+  const int64_t e0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256;
+  const int64_t e = e0 + threadIdx.x;
+  if (e < ne) {
+    // connectivity and coefficient are read once: nontemporal, so they do
+    // not evict the coordinates the neighbouring elements gather
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    const i4 cv = __builtin_nontemporal_load(reinterpret_cast<const i4*>(conn + e));
+    const int4 c = make_int4(cv.x, cv.y, cv.z, cv.w);
+    Geo G;
+    element_geometry(X, c, NPE - 1, G);
+    const double kv = __builtin_nontemporal_load(&kappa[e]) * G.meas;
+    const double m = mass_coeff * (signed_mass ? G.signed_meas : G.meas) * (NPE == 4 ? 1.0 / 20.0 : 1.0 / 12.0);
+    double* o = rec + threadIdx.x * P;
+    if (NPE == 4) {
+      int t = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int b = a + 1; b < 4; ++b)
-        v[t++] = kv * (G.g[a][0] * G.g[b][0] + G.g[a][1] * G.g[b][1] + G.g[a][2] * G.g[b][2]);
-    v[6] = m;
-    v[7] = 0.0;
-    double4* o4 = reinterpret_cast<double4*>(o);
-    o4[0] = make_double4(v[0], v[1], v[2], v[3]);
-    o4[1] = make_double4(v[4], v[5], v[6], v[7]);
-  } else {
-    const double s01 = kv * (G.g[0][0] * G.g[1][0] + G.g[0][1] * G.g[1][1]);
-    const double s02 = kv * (G.g[0][0] * G.g[2][0] + G.g[0][1] * G.g[2][1]);
-    const double s12 = kv * (G.g[1][0] * G.g[2][0] + G.g[1][1] * G.g[2][1]);
-    *reinterpret_cast<double4*>(o) = make_double4(s01, s02, s12, m);
+        for (int b = a + 1; b < 4; ++b)
+          o[t++] = kv * (G.g[a][0] * G.g[b][0] + G.g[a][1] * G.g[b][1] + G.g[a][2] * G.g[b][2]);
+      o[6] = m;
+      o[7] = 0.0;
+    } else {
+      o[0] = kv * (G.g[0][0] * G.g[1][0] + G.g[0][1] * G.g[1][1]);
+      o[1] = kv * (G.g[0][0] * G.g[2][0] + G.g[0][1] * G.g[2][1]);
+      o[2] = kv * (G.g[1][0] * G.g[2][0] + G.g[1][1] * G.g[2][1]);
+      o[3] = m;
+    }
+  }
+  __syncthreads();
+  const int64_t nrec = ne - e0 < 256 ? ne - e0 : 256;
+  double2* out = reinterpret_cast<double2*>(ev + e0 * S);
+  for (int i = threadIdx.x; i < nrec * (S / 2); i += 256) {
+    const int r = i / (S / 2), part = i % (S / 2);
+    out[i] = make_double2(rec[r * P + 2 * part], rec[r * P + 2 * part + 1]);
   }
 }
 
@@ -387,8 +403,8 @@ __device__ __forceinline__ void corner_row(int code, const int4* __restrict__ co
   corner_values<NPE>(a, r, val);
 }
 
-constexpr int kSliceSpan = 2048;  // LDS doubles per wavefront (16 KiB)
-constexpr int kUnroll = 2;        // incidences in flight per lane in assemble_slices_kernel
+constexpr int kSliceSpan = 2048;  // LDS doubles per slice (16 KiB)
+constexpr int kIncGroup = 4;      // incidences in flight per lane in assemble_slices_kernel
 
 // per slice of 64 rows: the most incidences of a row (0 for a slice that
 // takes the in-place path) and whether the slice takes that path
@@ -448,85 +464,111 @@ __global__ void inc_fill_kernel(int64_t nv, const int32_t* __restrict__ v2e_ptr,
   }
 }
 
-// rows of the slices that sum in LDS (see above)
-template <int NPE>
+// incidence read once per assembly: nontemporal, so it does not evict the
+// element records, which the rows of up to four slices read through L2
+__device__ __forceinline__ int2 load_inc(const int2* p) {
+  const long long v = __builtin_nontemporal_load(reinterpret_cast<const long long*>(p));
+  return make_int2((int)(v & 0xffffffffll), (int)(v >> 32));
+}
+
+// 64-bit value of quad lane J, broadcast to the four lanes of each quad (DPP)
+template <int J>
+__device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int kCtrl = J | (J << 2) | (J << 4) | (J << 6);  // quad_perm [J, J, J, J]
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), kCtrl, 0xf, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(b >> 32), kCtrl, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// rows of the slices that sum in LDS (see above).  A block of 256 threads
+// owns a slice, four lanes (a quad) per row.  For incidence (e, a) lane t
+// reads the one 8-byte record slot it needs, S_at for t != a and m on lane a,
+// so a record is read as one 64-byte line by a quad rather than whole by one
+// lane; the quad shares the four values by DPP broadcasts.  Lane t adds
+// S_at + m into the row's entry of column t in LDS, lane 0 sums the diagonal
+// contribution 2 m - sum_{b != a} S_ab in a register.  Every entry still sums
+// its elements in incidence order (the columns of one incidence are
+// distinct), with the values and order of corner_values.  With four lanes a
+// row a block holds one slice, so a CU has few slices in flight and the
+// records they share stay in L2 between their reads.
+template <int NPE, int U>
 __global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const int32_t* __restrict__ sptr,
                                                               const int32_t* __restrict__ is_long,
                                                               const int2* __restrict__ inc,
                                                               const double* __restrict__ ev,
                                                               const int32_t* __restrict__ rp,
-                                                              double* __restrict__ val, int span_max) {
-  extern __shared__ double lds[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                                                              double* __restrict__ val) {
+  extern __shared__ double acc[];
+  const int t = threadIdx.x & 3;
+  const int row = threadIdx.x >> 2;  // row of the slice
   const int64_t nsl = (nv + 63) / 64;
-  // XCD-contiguous block order: an element record serves the rows of its
-  // four vertices, which lie in the same or nearby slices, so each XCD's L2
-  // reads a contiguous range of records once
-  const int64_t sl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
-  double* acc = lds + wave * span_max;
-  // own: this wavefront sums its slice here (span <= span_max, rows <= 255)
-  const bool own = sl < nsl && is_long[sl] == 0;
-  int kn = 0, base = 0, span = 0;
-  int64_t o0 = 0;
-  if (own) {
+  const XcdRows R = xcd_rows(nsl, 1);
+  for (int64_t sl = R.first; sl < R.hi; sl += R.step) {
+    // slices with a wide span or a row over 255 entries sum in place elsewhere
+    if (is_long[sl]) continue;
     const int64_t r0 = sl * 64, r1 = r0 + 64 < nv ? r0 + 64 : nv;
-    o0 = rp[r0];
-    span = rp[r1] - (int)o0;
-    if (r0 + lane < nv) {  // lanes past the last row take no incidences
-      kn = sptr[sl + 1] - sptr[sl];
-      base = rp[r0 + lane] - (int)o0;
-    }
-    for (int t = lane; t < span; t += 64) acc[t] = 0.0;
-  }
-  __syncthreads();
-  if (kn > 0) {
-    const int2* q = inc + ((int64_t)sptr[sl] << 6) + lane;
-    // the diagonal sums in a register (every incidence adds to it), the
-    // off-diagonal entries in LDS; both in incidence order
+    const int64_t o0 = rp[r0];
+    const int span = rp[r1] - (int)o0;
+    // every lane runs the slice's incidence count (rows past the last one
+    // hold padding), so all lanes of a quad take part in the broadcasts
+    const int kn = sptr[sl + 1] - sptr[sl];
+    const int base = r0 + row < nv ? rp[r0 + row] - (int)o0 : 0;
+    for (int i = threadIdx.x; i < span; i += 256) acc[i] = 0.0;
+    __syncthreads();
+    const int2* q = inc + ((int64_t)sptr[sl] << 6) + row;
     double diag = 0.0;
     int dslot = -1;
-    auto add = [&](int2 ic, const double* r) {
-      const int a = ic.x & 3;
-      double v[4];
-      corner_values<NPE>(a, r, v);
+    // software pipeline over groups of U incidences: the next group's
+    // incidences load while this group's records are in flight; a padding
+    // incidence reads element 0's record and adds nothing
+    const int ng = (kn + U - 1) / U;
+    int2 nx[U];
+    auto inc_at = [&](int k) { return k < kn ? load_inc(q + ((int64_t)k << 6)) : make_int2(-1, 0); };
 #pragma unroll
-      for (int b = 0; b < NPE; ++b) {
-        const int sb = (ic.y >> (8 * b)) & 255;
-        if (b == a) {
-          diag += v[b];
-          dslot = sb;
-        } else {
-          acc[base + sb] += v[b];
+    for (int u = 0; u < U; ++u) nx[u] = inc_at(u);
+    for (int g = 0; g < ng; ++g) {
+      int2 ic[U];
+      double x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ic[u] = nx[u];
+        const int a = ic[u].x & 3;
+        const double* rec = ev + (int64_t)(ic[u].x >= 0 ? ic[u].x >> 2 : 0) * EvLayout<NPE>::kStride;
+        // lane t reads S_at, one 8-byte slot of the record; the corner's own
+        // lane (and lane 3 of a triangle) reads m, which the quad then shares
+        const int sl_t = (t == a || t >= NPE) ? EvLayout<NPE>::kM : EvLayout<NPE>::slot(a, t);
+        x[u] = rec[sl_t];
+      }
+      const int kk = (g + 1) * U;
+#pragma unroll
+      for (int u = 0; u < U; ++u) nx[u] = inc_at(kk + u);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int a = ic[u].x & 3;
+        // sum_{b != a} S_ab in ascending b, as corner_values adds it; m from lane a
+        double sum = 0.0;
+        const double y0 = quad_bcast<0>(x[u]), y1 = quad_bcast<1>(x[u]), y2 = quad_bcast<2>(x[u]);
+        const double y3 = quad_bcast<3>(x[u]);
+        if (a != 0) sum += y0;
+        if (a != 1) sum += y1;
+        if (a != 2) sum += y2;
+        if (NPE == 4 && a != 3) sum += y3;
+        const double m = a == 0 ? y0 : (a == 1 ? y1 : (a == 2 ? y2 : y3));
+        if (ic[u].x >= 0) {
+          if (t < NPE && t != a) acc[base + ((ic[u].y >> (8 * t)) & 255)] += x[u] + m;
+          if (t == 0) {
+            diag += 2.0 * m - sum;
+            dslot = (ic[u].y >> (8 * a)) & 255;
+          }
         }
       }
-    };
-    int k = 0;
-    // kUnroll incidences loaded at a time (independent load chains), added in order
-    for (; k + kUnroll - 1 < kn; k += kUnroll) {
-      int2 ic[kUnroll];
-      double r[kUnroll][8];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) ic[u] = q[(int64_t)(k + u) << 6];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        if (ic[u].x >= 0) load_record<NPE>(ev, ic[u].x >> 2, r[u]);
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        if (ic[u].x >= 0) add(ic[u], r[u]);
     }
-    for (; k < kn; ++k) {
-      const int2 i0 = q[(int64_t)k << 6];
-      if (i0.x >= 0) {
-        double r0[8];
-        load_record<NPE>(ev, i0.x >> 2, r0);
-        add(i0, r0);
-      }
-    }
-    if (dslot >= 0) acc[base + dslot] = diag;
+    if (t == 0 && dslot >= 0) acc[base + dslot] = diag;
+    __syncthreads();
+    for (int i = threadIdx.x; i < span; i += 256) __builtin_nontemporal_store(acc[i], val + o0 + i);
+    __syncthreads();
   }
-  __syncthreads();
-  if (own)
-    for (int t = lane; t < span; t += 64) val[o0 + t] = acc[t];
 }
 
 __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int c) {
@@ -866,14 +908,15 @@ void assemble_values(Handle& h) {
   ProfScope ps(h, "assembly", (double)h.ne * (16 + 8) + (double)h.nv * 32 + (double)h.A.nnz * 12 +
                                   (double)h.ne * npe * 4);
   const int64_t nsl = (h.nv + 63) / 64;
-  const unsigned ge = grid_for(h.ne, 256), gs = (unsigned)((nsl + 3) / 4);
-  const size_t lds = sizeof(double) * 4 * (size_t)h.asm_span;
+  const unsigned ge = grid_for(h.ne, 256);
+  const unsigned gs = (unsigned)nsl;
+  const size_t lds = sizeof(double) * (size_t)h.asm_span;
   if (npe == 4) {
     element_values_kernel<4><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
                                                 h.kappa.get(), ev);
     HIP_LAUNCH_CHECK();
-    assemble_slices_kernel<4><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(), h.asm_inc.get(), ev,
-                                                   rp, val, h.asm_span);
+    assemble_slices_kernel<4, kIncGroup><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(),
+                                                              h.asm_inc.get(), ev, rp, val);
     HIP_LAUNCH_CHECK();
     if (h.asm_nlong)
       assemble_rows_long_kernel<4><<<(unsigned)h.asm_nlong, 64, 0, s>>>(h.nv, h.asm_nlong, h.asm_long.get(),
@@ -883,8 +926,8 @@ void assemble_values(Handle& h) {
     element_values_kernel<3><<<ge, 256, 0, s>>>(h.ne, sm, h.mass_coeff, conn, (const double4*)h.coords.get(),
                                                 h.kappa.get(), ev);
     HIP_LAUNCH_CHECK();
-    assemble_slices_kernel<3><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(), h.asm_inc.get(), ev,
-                                                   rp, val, h.asm_span);
+    assemble_slices_kernel<3, kIncGroup><<<gs, 256, lds, s>>>(h.nv, h.asm_sptr.get(), h.asm_islong.get(),
+                                                              h.asm_inc.get(), ev, rp, val);
     HIP_LAUNCH_CHECK();
     if (h.asm_nlong)
       assemble_rows_long_kernel<3><<<(unsigned)h.asm_nlong, 64, 0, s>>>(h.nv, h.asm_nlong, h.asm_long.get(),
