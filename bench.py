@@ -46,11 +46,13 @@ def load_package():
 
 def solver_params(precision: int):
     # tuned V-cycle (sweeps at 161^3 over strength threshold 0-0.03, Chebyshev
-    # degree 2-5, prolongator weight 0.3-1.1, cycle type): Chebyshev(3)
-    # pre/post on D^-1 A, aggregation on the strength graph at theta = 0.01,
-    # prolongator smoothing weight 0.5, SA-AMG down to <= 256 rows
+    # degree 2-5, prolongator weight 0.3-1.1, cycle type, coarsest size):
+    # Chebyshev(3) pre/post on D^-1 A, aggregation on the strength graph at
+    # theta = 0.01, prolongator smoothing weight 0.5, SA-AMG until a level has
+    # fewer than 300 rows (at 161^3 the 256-row level is then the coarsest, one
+    # dense solve instead of a latency-bound level and a 26-row one below it)
     return dict(solverType=1, tolerance=1e-8, maxIters=500, precision=precision, smootherType=1,
-                preInnerIters=3, postInnerIters=3, postRelaxes=1, topSize=256, aggregationRule=1,
+                preInnerIters=3, postInnerIters=3, postRelaxes=1, topSize=300, aggregationRule=1,
                 strengthThreshold=0.01, proOmega=0.5)
 
 

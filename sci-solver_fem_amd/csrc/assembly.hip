@@ -482,16 +482,19 @@ __device__ __forceinline__ double quad_bcast(double v) {
 }
 
 // rows of the slices that sum in LDS (see above).  A block of 256 threads
-// owns a slice, four lanes (a quad) per row.  For incidence (e, a) lane t
-// reads the one 8-byte record slot it needs, S_at for t != a and m on lane a,
-// so a record is read as one 64-byte line by a quad rather than whole by one
-// lane; the quad shares the four values by DPP broadcasts.  Lane t adds
-// S_at + m into the row's entry of column t in LDS, lane 0 sums the diagonal
-// contribution 2 m - sum_{b != a} S_ab in a register.  Every entry still sums
-// its elements in incidence order (the columns of one incidence are
-// distinct), with the values and order of corner_values.  With four lanes a
-// row a block holds one slice, so a CU has few slices in flight and the
-// records they share stay in L2 between their reads.
+// owns a slice, four lanes (a quad) per row.  For incidence (e, a) lane t of
+// the quad reads 16-byte chunk t of the record, so the quad reads the record
+// as one contiguous 64-byte piece (a single lane reading it whole issues
+// four scattered 16-byte reads, and the address unit, not HBM, then bounds
+// the pass); lane t then takes the slot it needs from the lane holding it
+// (ds_bpermute): S_at for t != a, m on lane a, and the quad shares the four
+// values by DPP broadcasts.  Lane t adds S_at + m into the row's entry of
+// column t in LDS, lane 0 sums the diagonal contribution
+// 2 m - sum_{b != a} S_ab in a register.  Every entry still sums its
+// elements in incidence order (the columns of one incidence are distinct),
+// with the values and order of corner_values.  With four lanes a row a block
+// holds one slice, so a CU has few slices in flight and the records they
+// share stay in L2 between their reads.
 template <int NPE, int U>
 __global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const int32_t* __restrict__ sptr,
                                                               const int32_t* __restrict__ is_long,
@@ -529,16 +532,12 @@ __global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const 
     for (int u = 0; u < U; ++u) nx[u] = inc_at(u);
     for (int g = 0; g < ng; ++g) {
       int2 ic[U];
-      double x[U];
+      double2 ch[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         ic[u] = nx[u];
-        const int a = ic[u].x & 3;
         const double* rec = ev + (int64_t)(ic[u].x >= 0 ? ic[u].x >> 2 : 0) * EvLayout<NPE>::kStride;
-        // lane t reads S_at, one 8-byte slot of the record; the corner's own
-        // lane (and lane 3 of a triangle) reads m, which the quad then shares
-        const int sl_t = (t == a || t >= NPE) ? EvLayout<NPE>::kM : EvLayout<NPE>::slot(a, t);
-        x[u] = rec[sl_t];
+        ch[u] = reinterpret_cast<const double2*>(rec)[t % (EvLayout<NPE>::kStride / 2)];
       }
       const int kk = (g + 1) * U;
 #pragma unroll
@@ -546,17 +545,28 @@ __global__ void __launch_bounds__(256) assemble_slices_kernel(int64_t nv, const 
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int a = ic[u].x & 3;
+        // lane t takes S_at from the lane holding its slot (the corner's own
+        // lane, and lane 3 of a triangle, take m)
+        const int sl_t = (t == a || t >= NPE) ? EvLayout<NPE>::kM : EvLayout<NPE>::slot(a, t);
+        const int src = (((int)threadIdx.x & 63 & ~3) + (sl_t >> 1)) << 2;
+        const long long c0 = __double_as_longlong(ch[u].x), c1 = __double_as_longlong(ch[u].y);
+        const unsigned lo0 = __builtin_amdgcn_ds_bpermute(src, (int)(c0 & 0xffffffffll));
+        const unsigned hi0 = __builtin_amdgcn_ds_bpermute(src, (int)(c0 >> 32));
+        const unsigned lo1 = __builtin_amdgcn_ds_bpermute(src, (int)(c1 & 0xffffffffll));
+        const unsigned hi1 = __builtin_amdgcn_ds_bpermute(src, (int)(c1 >> 32));
+        const double x = __longlong_as_double((long long)((sl_t & 1) ? (((unsigned long long)hi1 << 32) | lo1)
+                                                                     : (((unsigned long long)hi0 << 32) | lo0)));
         // sum_{b != a} S_ab in ascending b, as corner_values adds it; m from lane a
         double sum = 0.0;
-        const double y0 = quad_bcast<0>(x[u]), y1 = quad_bcast<1>(x[u]), y2 = quad_bcast<2>(x[u]);
-        const double y3 = quad_bcast<3>(x[u]);
+        const double y0 = quad_bcast<0>(x), y1 = quad_bcast<1>(x), y2 = quad_bcast<2>(x);
+        const double y3 = quad_bcast<3>(x);
         if (a != 0) sum += y0;
         if (a != 1) sum += y1;
         if (a != 2) sum += y2;
         if (NPE == 4 && a != 3) sum += y3;
         const double m = a == 0 ? y0 : (a == 1 ? y1 : (a == 2 ? y2 : y3));
         if (ic[u].x >= 0) {
-          if (t < NPE && t != a) acc[base + ((ic[u].y >> (8 * t)) & 255)] += x[u] + m;
+          if (t < NPE && t != a) acc[base + ((ic[u].y >> (8 * t)) & 255)] += x + m;
           if (t == 0) {
             diag += 2.0 * m - sum;
             dslot = (ic[u].y >> (8 * a)) & 255;
