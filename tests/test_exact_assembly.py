@@ -81,3 +81,37 @@ def test_oracle_formula_loses_digits_with_refinement(oracle, meshgen, N):
     err_rounded = float(np.abs(v.astype(np.float64).astype(ea.LD) - v).max()) / scale
     assert err_rounded <= 1.2e-16
     assert err_oracle > 4 * err_rounded
+
+
+def _square(meshgen, N, sigma, seed=3):
+    xy, tris = meshgen.square_mesh(N, seed=seed)
+    return xy, tris, meshgen.lognormal_kappa(seed, tris.shape[0], sigma)
+
+
+def test_triangles_match_oracle_on_a_coarse_mesh(oracle, meshgen):
+    xy, tris, kappa = _square(meshgen, 5, 1.0)
+    rp, ci, ov = oracle.assemble(xy, tris, kappa, 0.3)
+    v = ea.assemble(xy, tris, kappa, 0.3, rp, ci).astype(np.float64)
+    assert np.abs(v - ov).max() <= 1e-13 * np.abs(ov).max()
+
+
+def test_triangles_harmonic_and_area(oracle, meshgen):
+    """Constant coefficient, mass 0: K u = 0 at interior rows for linear u;
+    coefficient 0, mass 1: the entries sum to the area of the unit square."""
+    N = 9
+    xy, tris, _ = _square(meshgen, N, 1.0)
+    rp, ci, _ = oracle.assemble(xy, tris, None, 0.0)
+    v = ea.assemble(xy, tris, np.full(tris.shape[0], 1.5), 0.0, rp, ci)
+    X = xy.astype(ea.LD)
+    u = ea.LD(0.7) * X[:, 0] - ea.LD(1.3) * X[:, 1] + ea.LD(0.5)
+    n = xy.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    Ku = np.zeros(n, dtype=ea.LD)
+    np.add.at(Ku, rows, v * u[ci])
+    interior = np.all((xy[:, :2] > 1e-12) & (xy[:, :2] < 1 - 1e-12), axis=1)
+    assert interior.sum() == (N - 2) ** 2
+    scale = float(np.abs(v).max() * np.abs(u).max())
+    assert float(np.abs(Ku[interior]).max()) <= 1e-17 * scale
+    assert float(np.abs(Ku[~interior]).max()) > 1e-3 * scale
+    m = ea.assemble(xy, tris, np.zeros(tris.shape[0]), 1.0, rp, ci)
+    assert abs(float(m.sum()) - 1.0) <= 1e-15

@@ -36,25 +36,26 @@ def test_pattern_and_values_match_oracle(fem, gpu, oracle, meshgen, dim, N, sigm
     s.close()
 
 
-@pytest.mark.parametrize("N,sigma", [(25, 2.0), (41, 2.0)])
-def test_values_match_extended_precision(fem, gpu, oracle, meshgen, N, sigma):
+@pytest.mark.parametrize("dim,N,sigma", [(3, 25, 2.0), (3, 41, 2.0), (2, 257, 1.5)])
+def test_values_match_extended_precision(fem, gpu, oracle, meshgen, dim, N, sigma):
     """Against the assembly in extended precision (tests/exact_assembly.py):
-    the GPU's FP64 values are within 1e-15 of max|K| (measured 3e-16, a few
-    FP64 roundings of the element terms an entry sums).  The oracle's FP64
-    restatement of the reference's formula (cofactors of absolute
+    the GPU's FP64 values are within 1e-15 of max|K| (measured 1e-16 to 3e-16,
+    a few FP64 roundings of the element terms an entry sums).  The oracle's
+    FP64 restatement of the reference's formula (cofactors of absolute
     coordinates) is 4e-14 to 7e-14 of max|K| from the same values on these
-    meshes and drifts further as the mesh is refined, which is what the 1e-13
-    bar against the oracle above allows for."""
+    tet meshes and 7e-12 on the 257^2 triangles, and drifts further as the
+    mesh is refined; the oracle comparisons above run on meshes coarse enough
+    for their 1e-13 bar."""
     import exact_assembly as ea
-    xyz, conn = meshgen.cube_mesh(N, seed=5)
+    xyz, conn = meshgen.cube_mesh(N, seed=5) if dim == 3 else meshgen.square_mesh(N, seed=5)
     kappa = meshgen.lognormal_kappa(5, conn.shape[0], sigma)
-    s, rp, ci, v = _gpu_matrix(fem, 3, xyz, conn, kappa, 1e-2)
+    s, rp, ci, v = _gpu_matrix(fem, dim, xyz, conn, kappa, 1e-2)
     ref = ea.assemble(xyz, conn, kappa, 1e-2, rp, ci)
     scale = float(np.abs(ref).max())
     err_gpu = float(np.abs(v.astype(ea.LD) - ref).max()) / scale
     _, _, ov = oracle.assemble(xyz, conn, kappa, 1e-2)
     err_oracle = float(np.abs(ov.astype(ea.LD) - ref).max()) / scale
-    report(f"[gpu] {N}^3 sigma={sigma}: |A_gpu - K_ext| / max|K| = {err_gpu:.2e}, oracle {err_oracle:.2e}")
+    report(f"[gpu] dim={dim} N={N} sigma={sigma}: |A_gpu - K_ext| / max|K| = {err_gpu:.2e}, oracle {err_oracle:.2e}")
     assert err_gpu <= 1e-15
     s.close()
 
