@@ -60,6 +60,48 @@ def test_values_match_extended_precision(fem, gpu, oracle, meshgen, dim, N, sigm
     s.close()
 
 
+def _hub_meshes(meshgen):
+    """Meshes whose hub vertex has a row longer than 255 entries (the slices
+    holding it sum in place in the CSR values): a 300^2 grid with a hub
+    joined to its 300 bottom-edge vertices by a fan of triangles, and the
+    cone of tets from an apex over every triangle of a planar grid (the apex
+    row holds all 22^2 grid vertices)."""
+    xy, tris = meshgen.square_mesh(300, seed=2)
+    bottom = np.flatnonzero(xy[:, 1] < 1e-12)
+    bottom = bottom[np.argsort(xy[bottom, 0])]
+    hub = xy.shape[0]
+    xy2 = np.vstack([xy, [[0.5, -0.5, 0.0]]])
+    fan = np.stack([np.full(bottom.size - 1, hub), bottom[1:], bottom[:-1]], 1)
+    tris2 = np.ascontiguousarray(np.concatenate([tris, fan]).astype(np.int32))
+    g, t = meshgen.square_mesh(22, seed=4)
+    apex = g.shape[0]
+    xyz3 = np.vstack([g, [[0.5, 0.5, 1.0]]])
+    tets3 = np.ascontiguousarray(np.concatenate([t, np.full((t.shape[0], 1), apex)], 1).astype(np.int32))
+    return [(2, xy2, tris2), (3, xyz3, tets3)]
+
+
+def test_long_rows_sum_in_place(fem, gpu, oracle, meshgen):
+    """Rows over 255 entries: pattern bit-exact, values within 1e-15 of
+    max|A| of the extended-precision assembly and within 1e-12 of the oracle
+    (whose FP64 formula is 6.3e-13 of max|A| off the extended-precision
+    values on the 300^2 grid), A bitwise symmetric."""
+    import exact_assembly as ea
+    import scipy.sparse as sp
+    for dim, xyz, conn in _hub_meshes(meshgen):
+        report(f"[gpu] assembly with a hub row, dim={dim}, {conn.shape[0]} elements")
+        kappa = meshgen.lognormal_kappa(9, conn.shape[0], 1.0)
+        s, rp, ci, v = _gpu_matrix(fem, dim, xyz, conn, kappa, 0.5)
+        assert np.diff(rp).max() > 255
+        orp, oci, ov = oracle.assemble(xyz, conn, kappa, 0.5)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert np.abs(v - ov).max() <= 1e-12 * np.abs(ov).max()
+        ref = ea.assemble(xyz, conn, kappa, 0.5, rp, ci)
+        assert float(np.abs(v.astype(ea.LD) - ref).max()) <= 1e-15 * float(np.abs(ref).max())
+        A = sp.csr_matrix((v, ci, rp), shape=(xyz.shape[0],) * 2)
+        assert np.count_nonzero((A - A.T).tocoo().data) == 0
+        s.close()
+
+
 def test_symmetric_and_deterministic(fem, gpu, meshgen):
     report("[gpu] assembly symmetry/determinism N=8")
     xyz, conn = meshgen.cube_mesh(8, seed=4)
