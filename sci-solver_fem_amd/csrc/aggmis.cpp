@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <thread>
 #include <vector>
 
 #include "fem_internal.hpp"
@@ -65,28 +66,73 @@ class MergeSplit {
   }
 
  private:
-  // sizes and the aggregate adjacency (sorted, distinct, no self)
+  // sizes and the aggregate adjacency (sorted, distinct, no self), in time
+  // linear in the graph: members grouped by aggregate, each aggregate's
+  // neighbours collected once (stamp) and sorted (a few tens each)
   void rebuild() {
-    size_.assign((size_t)nagg_, 0);
+    const size_t na = (size_t)nagg_;
+    size_.assign(na, 0);
     for (int64_t i = 0; i < n_; ++i) ++size_[(size_t)agg_[(size_t)i]];
-    std::vector<std::pair<int32_t, int32_t>> pairs;
-    pairs.reserve((size_t)rp_[n_]);
-    for (int64_t i = 0; i < n_; ++i) {
-      const int32_t a = agg_[(size_t)i];
-      for (int32_t q = rp_[i]; q < rp_[i + 1]; ++q) {
-        const int32_t b = agg_[(size_t)ci_[q]];
-        if (b != a) pairs.emplace_back(a, b);
+    std::vector<int64_t> mptr(na + 1, 0);
+    for (size_t a = 0; a < na; ++a) mptr[a + 1] = mptr[a] + size_[a];
+    std::vector<int32_t> members((size_t)n_);
+    {
+      std::vector<int64_t> at(mptr.begin(), mptr.end() - 1);
+      for (int64_t i = 0; i < n_; ++i) members[(size_t)at[(size_t)agg_[(size_t)i]]++] = (int32_t)i;
+    }
+    // contiguous ranges of aggregates per worker thread; each list depends on
+    // its aggregate only, so the result does not depend on the split
+    const int nt = workers(n_);
+    std::vector<std::vector<int32_t>> part((size_t)nt);
+    std::vector<int32_t> cnt(na, 0);
+    auto run = [&](int w) {
+      const size_t a0 = na * (size_t)w / (size_t)nt, a1 = na * (size_t)(w + 1) / (size_t)nt;
+      std::vector<int32_t> stamp(na, -1);
+      std::vector<int32_t>& out = part[(size_t)w];
+      for (size_t a = a0; a < a1; ++a) {
+        const size_t first = out.size();
+        for (int64_t t = mptr[a]; t < mptr[a + 1]; ++t) {
+          const int32_t i = members[(size_t)t];
+          for (int32_t q = rp_[i]; q < rp_[i + 1]; ++q) {
+            const int32_t b = agg_[(size_t)ci_[q]];
+            if (b != (int32_t)a && stamp[(size_t)b] != (int32_t)a) {
+              stamp[(size_t)b] = (int32_t)a;
+              out.push_back(b);
+            }
+          }
+        }
+        std::sort(out.begin() + (std::ptrdiff_t)first, out.end());
+        cnt[a] = (int32_t)(out.size() - first);
       }
+    };
+    parallel(nt, run);
+    gptr_.assign(na + 1, 0);
+    for (size_t a = 0; a < na; ++a) gptr_[a + 1] = gptr_[a] + cnt[a];
+    gadj_.resize((size_t)gptr_[na]);
+    size_t at = 0;
+    for (const auto& p : part) {
+      std::copy(p.begin(), p.end(), gadj_.begin() + (std::ptrdiff_t)at);
+      at += p.size();
     }
-    std::sort(pairs.begin(), pairs.end());
-    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
-    gptr_.assign((size_t)nagg_ + 1, 0);
-    gadj_.resize(pairs.size());
-    for (size_t t = 0; t < pairs.size(); ++t) {
-      ++gptr_[(size_t)pairs[t].first + 1];
-      gadj_[t] = pairs[t].second;
+  }
+
+  // worker threads for the linear passes: one per 64K nodes, at most the
+  // hardware threads and 16
+  static int workers(int64_t n) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hw, 16, n / 65536 + 1}));
+  }
+
+  template <typename Fn>
+  static void parallel(int nt, Fn&& fn) {
+    if (nt == 1) {
+      fn(0);
+      return;
     }
-    for (int64_t a = 0; a < nagg_; ++a) gptr_[(size_t)a + 1] += gptr_[(size_t)a];
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nt; ++w) pool.emplace_back(fn, w);
+    fn(0);
+    for (auto& t : pool) t.join();
   }
 
   bool out_of_bounds(int s) const { return s < lo_ || s > hi_; }
@@ -229,8 +275,13 @@ class MergeSplit {
     }
     std::vector<int32_t> result(agg_);
     const int64_t base = nagg_;
+    // each flagged aggregate splits on its own (reads the old aggregation,
+    // writes its members only): contiguous ranges of them per worker thread
+    const int nt = workers(n_);
+    auto run = [&](int w) {
+    const size_t k0 = order.size() * (size_t)w / (size_t)nt, k1 = order.size() * (size_t)(w + 1) / (size_t)nt;
     std::vector<int> lp, lj, dist, side, nd, ns;
-    for (size_t k = 0; k < order.size(); ++k) {
+    for (size_t k = k0; k < k1; ++k) {
       const int32_t cur = order[k];
       const int32_t fresh = (int32_t)(base + (int64_t)k);
       const int32_t* ids = members.data() + mptr[(size_t)cur];
@@ -315,6 +366,8 @@ class MergeSplit {
       for (int t = 0; t < m; ++t)
         if (side[(size_t)t] != -1) result[(size_t)ids[t]] = side[(size_t)t];
     }
+    };
+    parallel(nt, run);
     agg_.swap(result);
     nagg_ = base + (int64_t)order.size();
     rebuild();
