@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -123,16 +124,29 @@ class MergeSplit {
     return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hw, 16, n / 65536 + 1}));
   }
 
+  // fn(0 .. nt-1) on worker threads; an exception in a worker (allocation
+  // failure) is rethrown here after every worker has joined, and workers
+  // that cannot be started run on the calling thread
   template <typename Fn>
   static void parallel(int nt, Fn&& fn) {
-    if (nt == 1) {
-      fn(0);
-      return;
-    }
+    std::vector<std::exception_ptr> err((size_t)nt);
+    auto guarded = [&](int w) {
+      try {
+        fn(w);
+      } catch (...) {
+        err[(size_t)w] = std::current_exception();
+      }
+    };
     std::vector<std::thread> pool;
-    for (int w = 1; w < nt; ++w) pool.emplace_back(fn, w);
-    fn(0);
+    try {
+      for (int w = 1; w < nt; ++w) pool.emplace_back(guarded, w);
+    } catch (...) {
+    }
+    for (int w = 1 + (int)pool.size(); w < nt; ++w) guarded(w);
+    guarded(0);
     for (auto& t : pool) t.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
   }
 
   bool out_of_bounds(int s) const { return s < lo_ || s > hi_; }
