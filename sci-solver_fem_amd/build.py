@@ -17,7 +17,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "lib"
 LIB = OUT / "libfem_mi355x.so"
 SOURCES = ["primitives.hip", "assembly.hip", "sparse.hip", "amg_setup.hip", "solve.hip",
-           "mesh_io.cpp", "aggmis.cpp", "capi.cpp"]
+           "mesh_io.cpp", "aggmis.cpp", "devpool.cpp", "capi.cpp"]
 ARCH = os.environ.get("FEM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -84,8 +84,11 @@ void create_common(Handle& h, int dim, int64_t nv, const double* xyz, int64_t ne
   h.mass_coeff = mass;
   h.flags = flags;
   init_device(h);
-  build_mesh_device(h, xyz, conn, kappa);  // validates indices and NaNs on the device
-  HIP_CHECK(hipStreamSynchronize(h.stream));
+  {
+    DevPoolScope pool;  // the mesh build's temporaries reuse each other's blocks
+    build_mesh_device(h, xyz, conn, kappa);  // validates indices and NaNs on the device
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+  }
 }
 
 // internal CSR -> user-numbered CSR (host)
@@ -168,6 +171,7 @@ bool params_shape_changed(const fem_params& a, const fem_params& b) {
 void ensure_setup(Handle& h) {
   if (h.A.n == 0) fail(FEM_ERR_EMPTY, "Error no matrix specified");
   if (!h.H.built) {
+    DevPoolScope pool;  // the setup's temporaries reuse each other's blocks
     build_hierarchy(h);
   }
 }

@@ -32,6 +32,27 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 
 [[noreturn]] inline void fail(int status, const std::string& msg) { throw FemError(status, msg); }
 
+// Device allocations made on a thread while a DevPoolScope is alive there
+// come from the scope's size classes and go back to them when released, so a
+// phase with hundreds of temporaries (the AMG setup) neither waits for the
+// device at every hipFree nor calls hipMalloc for every temporary.  All work
+// of such a phase runs on one stream, which orders a block's reuse after its
+// earlier uses.  Blocks still held when the scope ends (buffers the phase
+// keeps) are freed by hipFree whenever their owner releases them.
+void* pool_alloc(size_t bytes);   // null when no scope is active on this thread
+bool pool_release(void* p);       // false when p is not a free-able pool block
+
+class DevPoolScope {
+ public:
+  DevPoolScope();
+  ~DevPoolScope();
+  DevPoolScope(const DevPoolScope&) = delete;
+  DevPoolScope& operator=(const DevPoolScope&) = delete;
+
+ private:
+  void* prev_;
+};
+
 // Owning device allocation.  Moves only.
 template <typename T>
 class DevBuf {
@@ -49,13 +70,16 @@ class DevBuf {
   void resize(size_t n) {
     if (n == n_) return;
     release();
-    if (n) HIP_CHECK(hipMalloc((void**)&p_, n * sizeof(T)));
+    if (n) {
+      p_ = static_cast<T*>(pool_alloc(n * sizeof(T)));
+      if (!p_) HIP_CHECK(hipMalloc((void**)&p_, n * sizeof(T)));
+    }
     n_ = n;
   }
   // grow-only variant for work areas
   void reserve(size_t n) { if (n > n_) resize(n); }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_ && !pool_release(p_)) (void)hipFree(p_);
     p_ = nullptr;
     n_ = 0;
   }
