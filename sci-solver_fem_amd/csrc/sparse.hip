@@ -26,6 +26,7 @@ namespace {
 constexpr int kSymBlock = 64;
 constexpr int kSymCap = 256;             // LDS hash slots per thread (power of two), long product rows
 constexpr int kSymCapSmall = 128;        // the same for products of ~100 or fewer terms per row
+constexpr double kStashBytes = 8e9;      // largest per-row column stash of the one-pass symbolic product
 constexpr int kNumBlock = 64;
 constexpr int kNumCap = 64;              // LDS accumulator slots per thread
 
@@ -136,6 +137,47 @@ __global__ void __launch_bounds__(kSymBlock) sym_lds_kernel(int64_t nrows, const
   compact_sort_small(tab, CAP, count);
   int o = c_rp[i];
   for (int t = 0; t < count; ++t) c_ci[o + t] = tab[t];
+}
+
+// One pass for rows that fit the LDS table: count the row's columns and keep
+// them, sorted, in a per-row stash of `stride` (= the table's fill limit)
+// slots; rows that overflow are marked with count -1 for the global path.
+template <int CAP>
+__global__ void __launch_bounds__(kSymBlock) sym_lds_stash_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                                const int32_t* __restrict__ l_ci,
+                                                                const int32_t* __restrict__ r_rp,
+                                                                const int32_t* __restrict__ r_ci,
+                                                                const int32_t* __restrict__ colmap, int drop_diag,
+                                                                int32_t* __restrict__ cnt, int32_t* __restrict__ stash,
+                                                                uint8_t* __restrict__ fit) {
+  constexpr int kLimit = CAP * 3 / 4;
+  __shared__ int lds[kSymBlock * CAP];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  int* tab = lds + threadIdx.x * CAP;
+  for (int t = 0; t < CAP; ++t) tab[t] = -1;
+  int count = 0;
+  const bool ok = row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, CAP - 1, count, kLimit);
+  fit[i] = ok ? 1 : 0;
+  if (!ok) {
+    cnt[i] = -1;
+    return;
+  }
+  compact_sort_small(tab, CAP, count);
+  int32_t* o = stash + i * kLimit;
+  for (int t = 0; t < count; ++t) o[t] = tab[t];
+  cnt[i] = count;
+}
+
+// the stashed rows into their CSR positions
+__global__ void stash_copy_kernel(int64_t nrows, int stride, const uint8_t* __restrict__ fit,
+                                  const int32_t* __restrict__ stash, const int32_t* __restrict__ c_rp,
+                                  int32_t* __restrict__ c_ci) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows || !fit[i]) return;
+  const int o = c_rp[i], len = c_rp[i + 1] - o;
+  const int32_t* src = stash + i * stride;
+  for (int t = 0; t < len; ++t) c_ci[o + t] = src[t];
 }
 
 __global__ void collect_overflow_kernel(int64_t nrows, const int32_t* __restrict__ cnt, const int32_t* __restrict__ l_rp,
@@ -860,7 +902,24 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
     small = ((double)L.nnz / (double)n) * ((double)rnnz / (double)r_rows) <= 128.0;
   }
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
-  if (n > 0) {
+  // rows that fit the LDS table keep their sorted columns from the counting
+  // pass in a stash of limit-sized slots (so the fill is a copy, not a second
+  // hashing pass) when the stash stays within kStashBytes
+  const int stride = (small ? kSymCapSmall : kSymCap) * 3 / 4;
+  const bool stashing = n > 0 && (double)n * stride * sizeof(int32_t) <= kStashBytes;
+  DevBuf<int32_t> stash;
+  DevBuf<uint8_t> fit;
+  if (stashing) {
+    stash.resize((size_t)n * stride);
+    fit.resize((size_t)n);
+    if (small)
+      sym_lds_stash_kernel<kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), stash.get(), fit.get());
+    else
+      sym_lds_stash_kernel<kSymCap><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
+          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), stash.get(), fit.get());
+    HIP_LAUNCH_CHECK();
+  } else if (n > 0) {
     if (small)
       sym_lds_kernel<false, kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
           n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), nullptr, nullptr);
@@ -896,7 +955,10 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   if (nnz >= ((int64_t)1 << 31)) fail(FEM_ERR_CAPACITY, "sparse product has 2^31 or more nonzeros");
   C.nnz = nnz;
   C.ci.resize(nnz > 0 ? nnz : 1);
-  if (n > 0) {
+  if (stashing) {
+    stash_copy_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, stride, fit.get(), stash.get(), C.rp.get(), C.ci.get());
+    HIP_LAUNCH_CHECK();
+  } else if (n > 0) {
     if (small)
       sym_lds_kernel<true, kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
           n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), C.rp.get(), C.ci.get());
