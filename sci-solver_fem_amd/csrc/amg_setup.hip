@@ -14,6 +14,7 @@
 // time() seed replaced by a SplitMix64 stream, so the aggregation is a
 // deterministic function of (graph, seed) and matches the oracle bit for bit.
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -167,9 +168,12 @@ __global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__
   if (add_to != -1) aggregated[i] = 1;
 }
 
+// one atomic per wavefront (ballot count) instead of one per node
 __global__ void count_unallocated_kernel(int64_t n, const int* __restrict__ aggregated, unsigned* __restrict__ cnt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && aggregated[i] == 0) atomicAdd(cnt, 1u);
+  const bool un = i < n && aggregated[i] == 0;
+  const unsigned k = (unsigned)__popcll(__ballot(un));
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(cnt, k);
 }
 
 __global__ void part_sizes_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ sizes) {
@@ -208,9 +212,13 @@ __global__ void stuck_assign_kernel(int64_t n, int base, const int32_t* __restri
   if (i < n && flag[i]) part[i] = base + excl[i];
 }
 
+// one atomic per wavefront (wave maximum first) instead of one per node
 __global__ void max_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ mx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(mx, part[i]);
+  int v = i < n ? part[i] : INT_MIN;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(mx, v);
 }
 
 unsigned read_counter(DevBuf<unsigned>& c, hipStream_t s) {
