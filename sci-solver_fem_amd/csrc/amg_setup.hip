@@ -168,12 +168,25 @@ __global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__
   if (add_to != -1) aggregated[i] = 1;
 }
 
-// one atomic per wavefront (ballot count) instead of one per node
-__global__ void count_unallocated_kernel(int64_t n, const int* __restrict__ aggregated, unsigned* __restrict__ cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool un = i < n && aggregated[i] == 0;
-  const unsigned k = (unsigned)__popcll(__ballot(un));
-  if ((threadIdx.x & 63) == 0 && k) atomicAdd(cnt, k);
+// Reductions to one device word: a capped grid strides over the array and
+// each block adds one atomic (atomics on one address serialise in L2, about
+// 10 ns each, so one per wavefront of a 4M-node pass costs 0.7 ms)
+constexpr int kReduceBlocks = 1024;
+
+__global__ void __launch_bounds__(256) count_unallocated_kernel(int64_t n, const int* __restrict__ aggregated,
+                                                                unsigned* __restrict__ cnt) {
+  __shared__ unsigned s_part[4];
+  unsigned k = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    k += aggregated[i] == 0 ? 1u : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off, 64);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (t) atomicAdd(cnt, t);
+  }
 }
 
 __global__ void part_sizes_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ sizes) {
@@ -212,14 +225,19 @@ __global__ void stuck_assign_kernel(int64_t n, int base, const int32_t* __restri
   if (i < n && flag[i]) part[i] = base + excl[i];
 }
 
-// one atomic per wavefront (wave maximum first) instead of one per node
-__global__ void max_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ mx) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int v = i < n ? part[i] : INT_MIN;
+__global__ void __launch_bounds__(256) max_kernel(int64_t n, const int* __restrict__ part, int* __restrict__ mx) {
+  __shared__ int s_part[4];
+  int v = INT_MIN;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v = max(v, part[i]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(mx, v);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(mx, max(max(s_part[0], s_part[1]), max(s_part[2], s_part[3])));
 }
+
+unsigned reduce_grid(int64_t n) { return std::min<unsigned>(grid_for(n, 256), (unsigned)kReduceBlocks); }
 
 unsigned read_counter(DevBuf<unsigned>& c, hipStream_t s) {
   unsigned v = 0;
@@ -708,14 +726,14 @@ void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int 
     HIP_LAUNCH_CHECK();
     std::swap(part_in, part_out);
     HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned), s));
-    count_unallocated_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, aggregated.get(), cnt.get());
+    count_unallocated_kernel<<<reduce_grid(n), 256, 0, s>>>(n, aggregated.get(), cnt.get());
     HIP_LAUNCH_CHECK();
     unsigned unalloc = read_counter(cnt, s);
     if (unalloc == 0) {
       // removeRuntyParts
       DevBuf<int> mx(1);
       dev_fill_i32(mx.get(), -1, 1, s);
-      max_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), mx.get());
+      max_kernel<<<reduce_grid(n), 256, 0, s>>>(n, part_in.get(), mx.get());
       HIP_LAUNCH_CHECK();
       int hmx = -1;
       HIP_CHECK(hipMemcpyAsync(&hmx, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -744,7 +762,7 @@ void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int 
       // aggregation ends (the reference loops forever here).
       DevBuf<int> mx(1);
       dev_fill_i32(mx.get(), -1, 1, s);
-      max_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, part_in.get(), mx.get());
+      max_kernel<<<reduce_grid(n), 256, 0, s>>>(n, part_in.get(), mx.get());
       HIP_LAUNCH_CHECK();
       int hmx = -1;
       HIP_CHECK(hipMemcpyAsync(&hmx, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
