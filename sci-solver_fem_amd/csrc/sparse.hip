@@ -441,36 +441,63 @@ __global__ void sell_window_range_kernel(int64_t n, int64_t nslices, int g, cons
   if (lane == 0) compact[sl] = fast ? 1 : (bmax >= 0 ? -1 : 0);
 }
 
-// Second pass, a thread per slice left undecided: the distinct blocks in order
-// of first appearance (rows in order, entries in row order); more than
+// Second pass, a wavefront per slice left undecided (lane = row position):
+// each lane collects its row's distinct blocks, then the wavefront takes the
+// union in ascending order, one wave-wide minimum per block; more than
 // kSellWindows of them: not compact.
 __global__ void sell_window_kernel(int64_t n, int64_t nslices, int g, const int32_t* __restrict__ rp,
                                    const int32_t* __restrict__ ci, const int32_t* __restrict__ srow,
                                    int32_t* __restrict__ swin, int32_t* __restrict__ compact) {
-  int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nslices || compact[sl] != -1) return;
+  const int64_t sl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (sl >= nslices || compact[sl] != -1) return;  // the whole wavefront
   const int rows = 64 / g;
-  int blk[kSellWindows];
-  int nb = 0;
-  bool ok = true;
-  for (int64_t p = sl * rows; p < sl * rows + rows && p < n && ok; ++p) {
+  int mine[kSellWindows + 1];
+  int nm = 0;
+  bool over = false;  // this row alone has more than kSellWindows blocks
+  const int64_t p = sl * rows + lane;
+  if (lane < rows && p < n) {
     const int64_t r = sell_row(srow, p);
-    for (int q = rp[r]; q < rp[r + 1]; ++q) {
+    for (int q = rp[r]; q < rp[r + 1] && !over; ++q) {
       const int b = ci[q] >> kSellWinBits;
       bool seen = false;
-      for (int i = 0; i < nb; ++i) seen = seen || blk[i] == b;
+      for (int t = 0; t < nm; ++t) seen = seen || mine[t] == b;
       if (seen) continue;
-      if (nb == kSellWindows) {
-        ok = false;
-        break;
-      }
-      blk[nb++] = b;
+      if (nm == kSellWindows) over = true;
+      else mine[nm++] = b;
     }
   }
+  bool ok = !__any(over);
+  unsigned taken = 0;  // bit t: mine[t] is in the union
+  int win[kSellWindows];
+  int nb = 0;
+  while (ok) {
+    int cand = 0x7fffffff;
+    for (int t = 0; t < nm; ++t)
+      if (!(taken >> t & 1u)) cand = min(cand, mine[t]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
+    if (cand == 0x7fffffff) break;
+    if (nb == kSellWindows) {
+      ok = false;
+      break;
+    }
+    win[nb++] = cand;
+    for (int t = 0; t < nm; ++t)
+      if (mine[t] == cand) taken |= 1u << t;
+  }
   ok = ok && nb > 0;
-  for (int i = 0; i < kSellWindows; ++i)
-    swin[sl * kSellWindows + i] = !ok ? -1 : (i < nb ? (blk[i] << kSellWinBits) : (blk[0] << kSellWinBits));
-  compact[sl] = ok ? 1 : 0;
+  if (lane < kSellWindows) {
+    int w = -1;
+    if (ok) {
+      w = win[0];
+      for (int t = 0; t < kSellWindows; ++t)
+        if (t == lane && t < nb) w = win[t];
+      w <<= kSellWinBits;
+    }
+    swin[sl * kSellWindows + lane] = w;
+  }
+  if (lane == 0) compact[sl] = ok ? 1 : 0;
 }
 
 // stored nonzeros of each compact slice (0 for the others)
@@ -1085,7 +1112,7 @@ void build_sell(Csr& A, bool with_f32, hipStream_t s, int64_t sort_window, bool 
     sell_window_range_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow,
                                                                     A.swin.get(), cflag.get());
     HIP_LAUNCH_CHECK();
-    sell_window_kernel<<<grid_for(ns, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow, A.swin.get(),
+    sell_window_kernel<<<grid_for(ns * 64, 256), 256, 0, s>>>(A.n, ns, lanes, A.rp.get(), A.ci.get(), srow, A.swin.get(),
                                                          cflag.get());
     HIP_LAUNCH_CHECK();
     A.ncompact = dev_exclusive_scan_i32(cflag.get(), cscan.get(), ns, s);
