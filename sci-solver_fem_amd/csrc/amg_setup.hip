@@ -581,7 +581,10 @@ __global__ void __launch_bounds__(kAggNT) agg_pattern_kernel(int64_t n, const in
   __shared__ int32_t buf[kAggNT * kAggRowCap];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int32_t* lst = buf + threadIdx.x * kAggRowCap;
+  // lane-interleaved lists (entry t of lane j at t * kAggNT + j): the lanes'
+  // scans and shifts address distinct banks
+  int32_t* const lb = buf + threadIdx.x;
+  auto lst = [lb](int t) -> int32_t& { return lb[t * kAggNT]; };
   const int s0 = rp[i], e0 = rp[i + 1];
   if (e0 - s0 > kAggRowCap) {
     if (!kFill) {
@@ -594,15 +597,15 @@ __global__ void __launch_bounds__(kAggNT) agg_pattern_kernel(int64_t n, const in
   for (int q = s0; q < e0; ++q) {
     const int a = agg[ci[q]];
     int pos = 0;
-    while (pos < c && lst[pos] < a) ++pos;
-    if (pos < c && lst[pos] == a) continue;
-    for (int t = c; t > pos; --t) lst[t] = lst[t - 1];
-    lst[pos] = a;
+    while (pos < c && lst(pos) < a) ++pos;
+    if (pos < c && lst(pos) == a) continue;
+    for (int t = c; t > pos; --t) lst(t) = lst(t - 1);
+    lst(pos) = a;
     ++c;
   }
   if (kFill) {
     const int o = pp[i];
-    for (int t = 0; t < c; ++t) pj[o + t] = lst[t];
+    for (int t = 0; t < c; ++t) pj[o + t] = lst(t);
   } else {
     cnt[i] = c;
   }

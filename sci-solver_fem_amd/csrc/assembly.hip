@@ -138,7 +138,8 @@ __global__ void lower_bound_kernel(int64_t nv, const uint32_t* __restrict__ keys
 }
 
 // Insert c into the ascending list buf[0..len); returns the new length (cap ends with -1).
-__device__ __forceinline__ int sorted_insert(int* buf, int len, int cap, int c) {
+template <typename Buf>
+__device__ __forceinline__ int sorted_insert(Buf buf, int len, int cap, int c) {
   int lo = 0, hi = len;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -151,19 +152,30 @@ __device__ __forceinline__ int sorted_insert(int* buf, int len, int cap, int c) 
   return len + 1;
 }
 
+struct LdsRow {
+  int* p;
+  __device__ __forceinline__ int& operator[](int t) const { return p[t * kRowBlock]; }
+};
+
 // Row pattern: sorted unique vertices of all elements incident to row vertex.
 // Pass 0 counts (cnt[row], or -1 when the row needs more than kRowCap slots),
 // pass 1 writes columns at rp[row].  Rows flagged -1 go through the global path.
+// With inc given (no row overflowed, slices laid out by sptr), pass 1 also
+// writes the row's assembly incidences (element code, positions of the
+// element's vertices in the row, one byte each) from the sorted row in LDS,
+// the same values as inc_fill_kernel, which searches the written columns.
 template <bool kFill>
 __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
     int64_t nv, int npe, const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
     const int4* __restrict__ conn, int32_t* __restrict__ cnt, const int32_t* __restrict__ rp,
-    int32_t* __restrict__ ci) {
+    int32_t* __restrict__ ci, const int32_t* __restrict__ sptr = nullptr, int2* __restrict__ inc = nullptr) {
   __shared__ int lds[kRowBlock * kRowCap];
   int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nv) return;
   if (kFill && cnt[row] < 0) return;
-  int* buf = lds + threadIdx.x * kRowCap;
+  // lane-interleaved rows (entry t of lane j at t * kRowBlock + j): the
+  // lanes' searches and shifts address distinct banks
+  const LdsRow buf{lds + threadIdx.x};
   int len = 0;
   len = sorted_insert(buf, len, kRowCap, (int)row);
   for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1] && len >= 0; ++p) {
@@ -176,6 +188,27 @@ __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
   } else {
     int32_t o = rp[row];
     for (int t = 0; t < len; ++t) ci[o + t] = buf[t];
+    if (inc && len >= 0) {
+      const int64_t sl = row >> 6;
+      const int lane = (int)(row & 63);
+      const int k0 = sptr[sl], kn = sptr[sl + 1] - k0;
+      // kn == 0: the slice sums in place and has no incidences
+      for (int p = v2e_ptr[row], k = 0; kn > 0 && p < v2e_ptr[row + 1]; ++p, ++k) {
+        const int code = v2e[p];
+        const int4 c = conn[code >> 2];
+        const int v[4] = {c.x, c.y, c.z, c.w};
+        unsigned slots = 0;
+        for (int b = 0; b < npe; ++b) {
+          int lo = 0, hi = len;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (buf[mid] < v[b]) lo = mid + 1; else hi = mid;
+          }
+          slots |= (unsigned)lo << (8 * b);
+        }
+        inc[((int64_t)(k0 + k) << 6) + lane] = make_int2(code, (int)slots);
+      }
+    }
   }
 }
 
@@ -724,6 +757,8 @@ int check_mesh_device(int64_t nconn, const int32_t* conn, int64_t nv, int64_t ne
   return f;
 }
 
+void plan_assembly_slices(Handle& h);
+
 void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const double* kappa) {
   hipStream_t s = h.stream;
   const int64_t nv = h.nv, ne = h.ne;
@@ -844,9 +879,15 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
     h.A.n = h.A.m = nv;
     h.A.nnz = nnz;
     h.A.ci.resize(nnz);
+    // without long rows every row is sorted in LDS by the fill pass, which
+    // then also writes the assembly incidences (no separate search pass)
+    const bool with_inc = nlong == 0;
+    if (with_inc) plan_assembly_slices(h);
     pattern_rows_kernel<true><<<grid_for(nv, kRowBlock), kRowBlock, 0, s>>>(
-        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), h.A.rp.get(), h.A.ci.get());
+        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), h.A.rp.get(), h.A.ci.get(),
+        with_inc ? h.asm_sptr.get() : nullptr, with_inc ? h.asm_inc.get() : nullptr);
     HIP_LAUNCH_CHECK();
+    if (with_inc) h.asm_ready = true;
     if (nlong) {
       pattern_rows_global_kernel<true><<<grid_for(nlong, 64), 64, 0, s>>>(
           nlong, long_rows.get(), off.get(), npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(),
@@ -861,8 +902,9 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
 }
 
 // Slice layout of the incidences for assemble_slices_kernel (once per
-// pattern; re-assembly with new coefficients reuses it).
-void build_assembly_slices(Handle& h) {
+// pattern; re-assembly with new coefficients reuses it): slice offsets, the
+// slices that sum in place, and the incidence array filled with padding.
+void plan_assembly_slices(Handle& h) {
   hipStream_t s = h.stream;
   const int64_t nv = h.nv, nsl = (nv + 63) / 64;
   DevBuf<int32_t> kmax(nsl), span(nsl), lg(nsl);
@@ -889,6 +931,13 @@ void build_assembly_slices(Handle& h) {
   h.asm_islong = std::move(lg);
   h.asm_inc.resize(tot > 0 ? (size_t)tot * 64 : 1);
   dev_fill_i32(reinterpret_cast<int32_t*>(h.asm_inc.get()), -1, 2 * h.asm_inc.size(), s);
+}
+
+// The layout and the incidences of a pattern built elsewhere (or with long rows).
+void build_assembly_slices(Handle& h) {
+  hipStream_t s = h.stream;
+  const int64_t nv = h.nv;
+  plan_assembly_slices(h);
   const int4* conn = (const int4*)h.conn.get();
   if (h.dim == 3)
     inc_fill_kernel<4><<<grid_for(nv, 256), 256, 0, s>>>(nv, h.v2e_ptr.get(), h.v2e.get(), conn, h.A.rp.get(),
