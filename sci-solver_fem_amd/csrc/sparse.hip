@@ -18,6 +18,7 @@
 #include "half.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fem {
 
@@ -27,9 +28,6 @@ constexpr int kSymBlock = 64;
 constexpr int kSymCap = 256;             // LDS hash slots per thread (power of two), long product rows
 constexpr int kSymCapSmall = 128;        // the same for products of ~100 or fewer terms per row
 constexpr double kStashBytes = 8e9;      // largest per-row column stash of the one-pass symbolic product
-constexpr int kNumBlock = 64;
-constexpr int kNumCap = 64;              // LDS accumulator slots per thread
-constexpr int kNumCapSmall = 32;         // the same for products whose rows fit it
 
 // A thread's table in LDS, lane-interleaved: entry t of thread j of a block
 // of B threads is element t * B + j, so the lanes of a wavefront address
@@ -246,62 +244,82 @@ __device__ __forceinline__ int bsearch_col(C cols, int len, int c) {
   return lo;
 }
 
-// Numeric product, a thread per row: the row's columns and accumulators in
-// LDS when the row has at most CAP columns (else in the output row itself);
-// every product term is added in (L entry, R entry) order.  CAP sets the
-// LDS per thread and with it the occupancy: 32 gives 2.7 times the
-// wavefronts per CU of 64.
-template <int CAP>
-__global__ void __launch_bounds__(kNumBlock) num_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
-                                                      const int32_t* __restrict__ l_ci, const double* __restrict__ l_v,
-                                                      const int32_t* __restrict__ r_rp, const int32_t* __restrict__ r_ci,
-                                                      const double* __restrict__ r_v, const int32_t* __restrict__ c_rp,
-                                                      const int32_t* __restrict__ c_ci, double* __restrict__ c_v) {
-  __shared__ int s_cols[kNumBlock * CAP];
-  __shared__ double s_acc[kNumBlock * CAP];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows) return;
-  const int o = c_rp[i];
-  const int len = c_rp[i + 1] - o;
-  // every term of row i added into acc at its column's position in cols
-  auto accumulate = [&](auto cols, auto acc) {
+// Numeric product, G lanes per row: the row's columns and accumulators in
+// LDS when the row has at most CAP columns.  The L entries of the row are
+// taken in order; for each, the lanes split the entries of the matching R
+// row, which fall on distinct output columns, so the lanes add into distinct
+// slots and every output value takes its terms in (L entry, R entry) order:
+// the order, and the result bit for bit, of one thread walking the row.
+// LDS ordering across the L entries: the group's lanes are in one wavefront,
+// whose LDS accesses execute in program order.  Rows longer than CAP are
+// summed in the output row itself by the group's first lane (global memory
+// gives no such order between lanes).  A table per row rather than per
+// thread keeps G times fewer tables in LDS: with one thread per row and 64
+// slots, 48 KB per wavefront left 3 wavefronts per CU.
+constexpr int kNumNT = 256;
+template <int G, int CAP>
+__global__ void __launch_bounds__(kNumNT) num_group_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+                                                          const int32_t* __restrict__ l_ci,
+                                                          const double* __restrict__ l_v,
+                                                          const int32_t* __restrict__ r_rp,
+                                                          const int32_t* __restrict__ r_ci,
+                                                          const double* __restrict__ r_v,
+                                                          const int32_t* __restrict__ c_rp,
+                                                          const int32_t* __restrict__ c_ci, double* __restrict__ c_v) {
+  constexpr int RPB = kNumNT / G;
+  constexpr int S = CAP + 1;  // padded row stride: the groups of a wavefront start on different banks
+  __shared__ int s_cols[RPB * S];
+  __shared__ double s_acc[RPB * S];
+  const int g = (int)threadIdx.x / G, lane = (int)threadIdx.x % G;
+  const int64_t i = (int64_t)blockIdx.x * RPB + g;
+  const bool live = i < nrows;
+  const int o = live ? c_rp[i] : 0;
+  const int len = live ? c_rp[i + 1] - o : 0;
+  const bool in_lds = len <= CAP;
+  int* cols = s_cols + g * S;
+  double* acc = s_acc + g * S;
+  if (in_lds)
+    for (int t = lane; t < len; t += G) {
+      cols[t] = c_ci[o + t];
+      acc[t] = 0.0;
+    }
+  __syncthreads();
+  if (live && in_lds) {
     for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
       const int k = l_ci[p];
       const double a = l_v[p];
-      for (int q = r_rp[k]; q < r_rp[k + 1]; ++q) acc[bsearch_col(cols, len, r_ci[q])] += a * r_v[q];
+      for (int q = r_rp[k] + lane; q < r_rp[k + 1]; q += G) acc[bsearch_col(cols, len, r_ci[q])] += a * r_v[q];
     }
-  };
-  if (len <= CAP) {
-    const LaneTab<int, kNumBlock> sc{s_cols + threadIdx.x};
-    const LaneTab<double, kNumBlock> acc{s_acc + threadIdx.x};
-    for (int t = 0; t < len; ++t) {
-      sc[t] = c_ci[o + t];
-      acc[t] = 0.0;
+  }
+  __syncthreads();
+  if (in_lds)
+    for (int t = lane; t < len; t += G) c_v[o + t] = acc[t];
+  if (live && !in_lds && lane == 0) {
+    const int32_t* gc = c_ci + o;
+    double* ga = c_v + o;
+    for (int t = 0; t < len; ++t) ga[t] = 0.0;
+    for (int p = l_rp[i]; p < l_rp[i + 1]; ++p) {
+      const int k = l_ci[p];
+      const double a = l_v[p];
+      for (int q = r_rp[k]; q < r_rp[k + 1]; ++q) ga[bsearch_col(gc, len, r_ci[q])] += a * r_v[q];
     }
-    accumulate(sc, acc);
-    for (int t = 0; t < len; ++t) c_v[o + t] = acc[t];
-  } else {
-    double* acc = c_v + o;
-    for (int t = 0; t < len; ++t) acc[t] = 0.0;
-    accumulate(c_ci + o, acc);
   }
 }
 
-// rows of a CSR matrix with more than `cap` entries (a capped grid, one
-// atomic per block)
-__global__ void __launch_bounds__(256) rows_longer_kernel(int64_t n, const int32_t* __restrict__ rp, int cap,
-                                                        unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long s_part[4];
-  unsigned long long c = 0;
+// longest row of a CSR matrix (a capped grid, one atomic per block)
+__global__ void __launch_bounds__(256) max_row_len_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                        int* __restrict__ out) {
+  __shared__ int s_part[4];
+  int m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    c += (rp[i + 1] - rp[i] > cap) ? 1ull : 0ull;
+    m = max(m, rp[i + 1] - rp[i]);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = c;
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned long long t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
-    if (t) atomicAdd(out, t);
+    const int t = max(max(s_part[0], s_part[1]), max(s_part[2], s_part[3]));
+    if (t) atomicMax(out, t);
   }
 }
 
@@ -311,7 +329,7 @@ __global__ void __launch_bounds__(256) rows_longer_kernel(int64_t n, const int32
 // in ascending order.  Numeric: the L entries of the row are taken in order;
 // for each, the lanes split the entries of the matching R row, which hit
 // distinct output columns, so every output value is summed in the same order
-// as in num_kernel and the result is bitwise the same.
+// as in num_group_kernel and the result is bitwise the same.
 constexpr int kWaveBitmapCols = 1 << 17;  // 16 KB bitmap per row
 constexpr int kWaveAccCap = 2048;         // LDS accumulators per row (16 KB)
 
@@ -1090,24 +1108,42 @@ void spgemm_numeric(const Csr& L, const Csr& Rm, Csr& C, hipStream_t s) {
     HIP_LAUNCH_CHECK();
     return;
   }
-  // the small table when at most one row in a thousand overflows it (the
-  // fine-level A P at 161^3: 3.6 ms against 5.7 ms with 64 slots)
-  DevBuf<unsigned long long> longer(1);
-  HIP_CHECK(hipMemsetAsync(longer.get(), 0, sizeof(unsigned long long), s));
-  rows_longer_kernel<<<std::min<unsigned>(grid_for(C.n, 256), 1024u), 256, 0, s>>>(C.n, C.rp.get(), kNumCapSmall,
-                                                                                   longer.get());
+  // table size: the longest row, up to 256 (longer rows sum in global
+  // memory); lanes per row: the R rows' average length
+  DevBuf<int> mx(1);
+  HIP_CHECK(hipMemsetAsync(mx.get(), 0, sizeof(int), s));
+  max_row_len_kernel<<<std::min<unsigned>(grid_for(C.n, 256), 1024u), 256, 0, s>>>(C.n, C.rp.get(), mx.get());
   HIP_LAUNCH_CHECK();
-  unsigned long long nl = 0;
-  HIP_CHECK(hipMemcpyAsync(&nl, longer.get(), sizeof(nl), hipMemcpyDeviceToHost, s));
+  int maxlen = 0;
+  HIP_CHECK(hipMemcpyAsync(&maxlen, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  if (nl * 1000 <= (unsigned long long)C.n)
-    num_kernel<kNumCapSmall><<<grid_for(C.n, kNumBlock), kNumBlock, 0, s>>>(
-        C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(), Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(),
-        C.v64.get());
-  else
-    num_kernel<kNumCap><<<grid_for(C.n, kNumBlock), kNumBlock, 0, s>>>(
-        C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(), Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(),
-        C.v64.get());
+  const double ravg = Rm.n > 0 ? (double)Rm.nnz / (double)Rm.n : 0.0;
+  const int cap = maxlen <= 32 ? 32 : (maxlen <= 64 ? 64 : (maxlen <= 128 ? 128 : 256));
+  // at least cap / 16 lanes per row: a block's tables stay within 64 KB
+  const int g = std::max(ravg <= 4.0 ? 4 : (ravg <= 8.0 ? 8 : 16), cap / 16);
+  auto launch = [&](auto gc, auto capc) {
+    constexpr int G = decltype(gc)::value, CAP = decltype(capc)::value;
+    if constexpr ((size_t)(kNumNT / G) * (CAP + 1) * 12 <= 65536) {
+      const unsigned nb = (unsigned)((C.n + kNumNT / G - 1) / (kNumNT / G));
+      num_group_kernel<G, CAP><<<nb, kNumNT, 0, s>>>(C.n, L.rp.get(), L.ci.get(), L.v64.get(), Rm.rp.get(),
+                                                     Rm.ci.get(), Rm.v64.get(), C.rp.get(), C.ci.get(), C.v64.get());
+    } else {
+      fail(FEM_ERR_STATE, "numeric sparse product: table size and lanes per row out of range");
+    }
+  };
+  auto by_cap = [&](auto gc) {
+    switch (cap) {
+      case 32: launch(gc, std::integral_constant<int, 32>{}); break;
+      case 64: launch(gc, std::integral_constant<int, 64>{}); break;
+      case 128: launch(gc, std::integral_constant<int, 128>{}); break;
+      default: launch(gc, std::integral_constant<int, 256>{}); break;
+    }
+  };
+  switch (g) {
+    case 4: by_cap(std::integral_constant<int, 4>{}); break;
+    case 8: by_cap(std::integral_constant<int, 8>{}); break;
+    default: by_cap(std::integral_constant<int, 16>{}); break;
+  }
   HIP_LAUNCH_CHECK();
 }
 
