@@ -153,34 +153,94 @@ __global__ void __launch_bounds__(kSymBlock) sym_lds_kernel(int64_t nrows, const
   for (int t = 0; t < count; ++t) c_ci[o + t] = tab[t];
 }
 
-// One pass for rows that fit the LDS table: count the row's columns and keep
-// them, sorted, in a per-row stash of `stride` (= the table's fill limit)
-// slots; rows that overflow are marked with count -1 for the global path.
-template <int CAP>
-__global__ void __launch_bounds__(kSymBlock) sym_lds_stash_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
+// One pass for rows that fit an LDS table of CAP slots: G lanes per row
+// insert the row's product columns into the row's hash table (LDS atomics:
+// the lanes of a row split the entries of each R row, or the L entries when
+// there is no R), then compact the table, rank-sort it and keep the row's
+// sorted columns in a per-row stash of CAP * 3 / 4 slots.  The set of
+// columns, its count and whether it exceeds the table's fill limit do not
+// depend on the insertion order, so the result is deterministic; rows over
+// the limit are marked with count -1 for the global path.  A table per row
+// rather than per thread keeps G times fewer tables in LDS (one thread per
+// row with 256 slots held a wavefront's 64 tables in 64 KB).
+constexpr int kSymNT = 256;
+template <int G, int CAP>
+__global__ void __launch_bounds__(kSymNT) sym_group_stash_kernel(int64_t nrows, const int32_t* __restrict__ l_rp,
                                                                 const int32_t* __restrict__ l_ci,
                                                                 const int32_t* __restrict__ r_rp,
                                                                 const int32_t* __restrict__ r_ci,
                                                                 const int32_t* __restrict__ colmap, int drop_diag,
                                                                 int32_t* __restrict__ cnt, int32_t* __restrict__ stash,
                                                                 uint8_t* __restrict__ fit) {
+  constexpr int RPB = kSymNT / G;
+  constexpr int S = CAP + 1;  // padded row stride: the groups of a wavefront start on different banks
   constexpr int kLimit = CAP * 3 / 4;
-  __shared__ int lds[kSymBlock * CAP];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows) return;
-  const LaneTab<int, kSymBlock> tab{lds + threadIdx.x};
-  for (int t = 0; t < CAP; ++t) tab[t] = -1;
-  int count = 0;
-  const bool ok = row_columns(i, l_rp, l_ci, r_rp, r_ci, colmap, drop_diag != 0, tab, CAP - 1, count, kLimit);
-  fit[i] = ok ? 1 : 0;
-  if (!ok) {
-    cnt[i] = -1;
-    return;
+  __shared__ int s_tab[RPB * S];
+  __shared__ int s_cnt[RPB];
+  __shared__ int s_over[RPB];
+  const int g = (int)threadIdx.x / G, lane = (int)threadIdx.x % G;
+  const int64_t i = (int64_t)blockIdx.x * RPB + g;
+  const bool live = i < nrows;
+  int* tab = s_tab + g * S;
+  for (int t = lane; t < CAP; t += G) tab[t] = -1;
+  if (lane == 0) {
+    s_cnt[g] = 0;
+    s_over[g] = 0;
   }
-  compact_sort_small(tab, CAP, count);
+  __syncthreads();
+  if (live) {
+    auto insert = [&](int c) {
+      if (drop_diag && c == (int)i) return;
+      unsigned p = hash_slot(c, CAP - 1);
+      for (int probe = 0; probe < CAP; ++probe) {
+        const int old = atomicCAS(&tab[p], -1, c);
+        if (old == c) return;
+        if (old == -1) {
+          if (atomicAdd(&s_cnt[g], 1) >= kLimit) s_over[g] = 1;
+          return;
+        }
+        p = (p + 1) & (CAP - 1);
+      }
+      s_over[g] = 1;  // full table (more keys than the limit)
+    };
+    if (r_rp) {
+      for (int p = l_rp[i]; p < l_rp[i + 1] && !s_over[g]; ++p) {
+        const int k = l_ci[p];
+        for (int q = r_rp[k] + lane; q < r_rp[k + 1]; q += G) insert(colmap ? colmap[r_ci[q]] : r_ci[q]);
+      }
+    } else {
+      for (int p = l_rp[i] + lane; p < l_rp[i + 1] && !s_over[g]; p += G) {
+        const int k = l_ci[p];
+        insert(colmap ? colmap[k] : k);
+      }
+    }
+  }
+  __syncthreads();
+  const bool ok = live && !s_over[g];
+  const int count = s_cnt[g];
+  if (live && lane == 0) {
+    fit[i] = ok ? 1 : 0;
+    cnt[i] = ok ? count : -1;
+  }
+  if (!ok) return;  // the whole group leaves
+  // compact in place: a chunk's keys are all read (one load instruction of
+  // the group) before any is written, and writes land at or below the chunk
+  const int gbase = (int)(threadIdx.x & 63) & ~(G - 1);
+  int base = 0;
+  for (int t0 = 0; t0 < CAP; t0 += G) {
+    const int key = tab[t0 + lane];
+    const unsigned gm = (unsigned)((__ballot(key != -1) >> gbase) & ((1ull << G) - 1));
+    if (key != -1) tab[base + __popc(gm & ((1u << lane) - 1))] = key;
+    base += __popc(gm);
+  }
+  // rank sort (distinct keys)
   int32_t* o = stash + i * kLimit;
-  for (int t = 0; t < count; ++t) o[t] = tab[t];
-  cnt[i] = count;
+  for (int t = lane; t < count; t += G) {
+    const int key = tab[t];
+    int r = 0;
+    for (int u = 0; u < count; ++u) r += tab[u] < key;
+    o[r] = key;
+  }
 }
 
 // the stashed rows into their CSR positions
@@ -993,7 +1053,6 @@ bool use_wave_rows(int64_t nrows, int64_t ncols) { return nrows > 0 && nrows <= 
 
 void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int64_t r_rows, const int32_t* colmap,
                      int64_t ncols, bool drop_diag, Csr& C, hipStream_t s) {
-  (void)r_rows;
   const int64_t n = L.n;
   C.n = n;
   C.m = ncols;
@@ -1018,11 +1077,13 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   // the LDS per thread (twice the blocks per CU) when rows are short; rows
   // that do not fit take the global-memory path either way
   bool small = true;
+  double ravg = 0.0;  // average R row length
   if (r_rp && n > 0 && r_rows > 0) {
     int32_t rnnz = 0;
     HIP_CHECK(hipMemcpyAsync(&rnnz, r_rp + r_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    small = ((double)L.nnz / (double)n) * ((double)rnnz / (double)r_rows) <= 128.0;
+    ravg = (double)rnnz / (double)r_rows;
+    small = ((double)L.nnz / (double)n) * ravg <= 128.0;
   }
   DevBuf<int32_t> cnt(n > 0 ? n : 1);
   // rows that fit the LDS table keep their sorted columns from the counting
@@ -1035,12 +1096,30 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   if (stashing) {
     stash.resize((size_t)n * stride);
     fit.resize((size_t)n);
-    if (small)
-      sym_lds_stash_kernel<kSymCapSmall><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
-          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), stash.get(), fit.get());
-    else
-      sym_lds_stash_kernel<kSymCap><<<grid_for(n, kSymBlock), kSymBlock, 0, s>>>(
-          n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap, drop_diag ? 1 : 0, cnt.get(), stash.get(), fit.get());
+    // lanes per row from the length of what they split (the R rows, else
+    // the L rows); at least 8 with 256 slots (a block's tables within 64 KB)
+    const double split = r_rp ? ravg : (double)L.nnz / (double)n;
+    int g = split <= 4.0 ? 4 : (split <= 8.0 ? 8 : 16);
+    if (!small) g = std::max(g, 8);
+    auto launch = [&](auto gc, auto capc) {
+      constexpr int G = decltype(gc)::value, CAP = decltype(capc)::value;
+      const unsigned nb = (unsigned)((n + kSymNT / G - 1) / (kSymNT / G));
+      sym_group_stash_kernel<G, CAP><<<nb, kSymNT, 0, s>>>(n, L.rp.get(), L.ci.get(), r_rp, r_ci, colmap,
+                                                           drop_diag ? 1 : 0, cnt.get(), stash.get(), fit.get());
+    };
+    using Small = std::integral_constant<int, kSymCapSmall>;
+    using Large = std::integral_constant<int, kSymCap>;
+    switch (g) {
+      case 4: launch(std::integral_constant<int, 4>{}, Small{}); break;  // g = 4 only with the small table
+      case 8:
+        if (small) launch(std::integral_constant<int, 8>{}, Small{});
+        else launch(std::integral_constant<int, 8>{}, Large{});
+        break;
+      default:
+        if (small) launch(std::integral_constant<int, 16>{}, Small{});
+        else launch(std::integral_constant<int, 16>{}, Large{});
+        break;
+    }
     HIP_LAUNCH_CHECK();
   } else if (n > 0) {
     if (small)
