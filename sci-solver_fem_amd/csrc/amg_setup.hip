@@ -316,25 +316,52 @@ __global__ void diag_kernel(int64_t n, const int32_t* __restrict__ rp, const int
 
 // P(i, J) = delta(J, agg(i)) - omega / d_i * sum_{k: agg(k) = J} a_ik, summed in
 // ascending k then the tentative 1 (order of the reference's reduce_by_key).
-__global__ void prolongator_kernel(int64_t n, const int32_t* __restrict__ a_rp, const int32_t* __restrict__ a_ci,
-                                   const double* __restrict__ a_v, const double* __restrict__ dinv,
-                                   const int32_t* __restrict__ agg, double omega, const int32_t* __restrict__ p_rp,
-                                   const int32_t* __restrict__ p_ci, double* __restrict__ p_v) {
+// Rows of at most kProRowCap entries accumulate in LDS, lane-interleaved
+// (entry t of lane j at t * kProNT + j), with the columns searched there;
+// longer rows accumulate in place.  Either way each entry takes its terms in
+// the order of the row of A, then the tentative 1.
+constexpr int kProNT = 64;
+constexpr int kProRowCap = 32;
+__global__ void __launch_bounds__(kProNT) prolongator_kernel(int64_t n, const int32_t* __restrict__ a_rp,
+                                                           const int32_t* __restrict__ a_ci,
+                                                           const double* __restrict__ a_v,
+                                                           const double* __restrict__ dinv,
+                                                           const int32_t* __restrict__ agg, double omega,
+                                                           const int32_t* __restrict__ p_rp,
+                                                           const int32_t* __restrict__ p_ci, double* __restrict__ p_v) {
+  __shared__ int s_c[kProRowCap * kProNT];
+  __shared__ double s_v[kProRowCap * kProNT];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int o = p_rp[i], len = p_rp[i + 1] - o;
-  for (int t = 0; t < len; ++t) p_v[o + t] = 0.0;
   const double d = (dinv[i] != 0.0) ? 1.0 / dinv[i] : 0.0;
-  auto slot = [&](int J) {
-    int lo = 0, hi = len;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (p_ci[o + mid] < J) lo = mid + 1; else hi = mid;
-    }
-    return o + lo;
+  auto row = [&](auto cols, auto vals) {
+    auto slot = [&](int J) {
+      int lo = 0, hi = len;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols(mid) < J) lo = mid + 1; else hi = mid;
+      }
+      return lo;
+    };
+    for (int q = a_rp[i]; q < a_rp[i + 1]; ++q) vals(slot(agg[a_ci[q]])) += (-omega * a_v[q]) / d;
+    vals(slot(agg[i])) += 1.0;
   };
-  for (int q = a_rp[i]; q < a_rp[i + 1]; ++q) p_v[slot(agg[a_ci[q]])] += (-omega * a_v[q]) / d;
-  p_v[slot(agg[i])] += 1.0;
+  if (len <= kProRowCap) {
+    int* const lc = s_c + threadIdx.x;
+    double* const lv = s_v + threadIdx.x;
+    for (int t = 0; t < len; ++t) {
+      lc[t * kProNT] = p_ci[o + t];
+      lv[t * kProNT] = 0.0;
+    }
+    row([lc](int t) -> int& { return lc[t * kProNT]; }, [lv](int t) -> double& { return lv[t * kProNT]; });
+    for (int t = 0; t < len; ++t) p_v[o + t] = lv[t * kProNT];
+  } else {
+    for (int t = 0; t < len; ++t) p_v[o + t] = 0.0;
+    const int32_t* gc = p_ci + o;
+    double* gv = p_v + o;
+    row([gc](int t) -> const int32_t& { return gc[t]; }, [gv](int t) -> double& { return gv[t]; });
+  }
 }
 
 __global__ void dinv_scale_kernel(int64_t n, const double* __restrict__ dinv, double* __restrict__ w) {
@@ -1053,9 +1080,9 @@ void build_hierarchy(Handle& h) {
     if (!agg_pattern(A, L.agg.get(), nagg, L.P, s))
       spgemm_symbolic(A, nullptr, nullptr, 0, L.agg.get(), nagg, false, L.P, s);
     L.P.v64.resize(L.P.nnz);
-    prolongator_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(), L.dinv64.get(),
-                                                        L.agg.get(), omega, L.P.rp.get(), L.P.ci.get(),
-                                                        L.P.v64.get());
+    prolongator_kernel<<<grid_for(n, kProNT), kProNT, 0, s>>>(n, A.rp.get(), A.ci.get(), A.v64.get(),
+                                                              L.dinv64.get(), L.agg.get(), omega, L.P.rp.get(),
+                                                              L.P.ci.get(), L.P.v64.get());
     HIP_LAUNCH_CHECK();
     csr_transpose(L.P, L.R, s);
     // A_c = R (A P)

@@ -243,15 +243,18 @@ __global__ void __launch_bounds__(kSymNT) sym_group_stash_kernel(int64_t nrows, 
   }
 }
 
-// the stashed rows into their CSR positions
+// the stashed rows into their CSR positions, kCopyLanes lanes per row
+constexpr int kCopyLanes = 16;
 __global__ void stash_copy_kernel(int64_t nrows, int stride, const uint8_t* __restrict__ fit,
                                   const int32_t* __restrict__ stash, const int32_t* __restrict__ c_rp,
                                   int32_t* __restrict__ c_ci) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = gid / kCopyLanes;
+  const int t = (int)(gid % kCopyLanes);
   if (i >= nrows || !fit[i]) return;
   const int o = c_rp[i], len = c_rp[i + 1] - o;
   const int32_t* src = stash + i * stride;
-  for (int t = 0; t < len; ++t) c_ci[o + t] = src[t];
+  for (int q = t; q < len; q += kCopyLanes) c_ci[o + q] = src[q];
 }
 
 __global__ void collect_overflow_kernel(int64_t nrows, const int32_t* __restrict__ cnt, const int32_t* __restrict__ l_rp,
@@ -1158,7 +1161,8 @@ void spgemm_symbolic(const Csr& L, const int32_t* r_rp, const int32_t* r_ci, int
   C.nnz = nnz;
   C.ci.resize(nnz > 0 ? nnz : 1);
   if (stashing) {
-    stash_copy_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, stride, fit.get(), stash.get(), C.rp.get(), C.ci.get());
+    stash_copy_kernel<<<grid_for(n * kCopyLanes, 256), 256, 0, s>>>(n, stride, fit.get(), stash.get(), C.rp.get(),
+                                                                     C.ci.get());
     HIP_LAUNCH_CHECK();
   } else if (n > 0) {
     if (small)
