@@ -364,15 +364,20 @@ __global__ void __launch_bounds__(kProNT) prolongator_kernel(int64_t n, const in
   }
 }
 
-__global__ void dinv_scale_kernel(int64_t n, const double* __restrict__ dinv, double* __restrict__ w) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] *= dinv[i];
-}
-
-__global__ void sumsq_kernel(int64_t n, const double* __restrict__ v, double* __restrict__ part) {
+// per-block partial sums of v_i^2; with dinv, v is first scaled in place
+// (v_i *= dinv_i) and the sum taken over the scaled values
+__global__ void sumsq_kernel(int64_t n, double* __restrict__ v, const double* __restrict__ dinv,
+                             double* __restrict__ part) {
   __shared__ double s_buf[kNT];
   double a = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) a += v[i] * v[i];
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
+    double x = v[i];
+    if (dinv) {
+      x *= dinv[i];
+      v[i] = x;
+    }
+    a += x * x;
+  }
   double t = block_sum(a, s_buf);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -392,7 +397,16 @@ __global__ void hash_vector_kernel(int64_t n, unsigned long long seed, double* _
 __global__ void norm_finish_kernel(int nb, const double* __restrict__ part, double* __restrict__ nrm) {
   if (threadIdx.x != 0) return;
   double t = 0.0;
-  for (int b = 0; b < nb; ++b) t += part[b];
+  // in index order; the loads of a chunk are issued together
+  int b = 0;
+  for (; b + 16 <= nb; b += 16) {
+    double c[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c[j] = part[b + j];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t += c[j];
+  }
+  for (; b < nb; ++b) t += part[b];
   nrm[0] = sqrt(t);
 }
 
@@ -411,20 +425,18 @@ double estimate_rho(const Csr& A, const double* dinv, hipStream_t s, int iters =
   DevBuf<double> v(n), w(n), part(nb), nrm(iters + 1);
   hash_vector_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, 12345ull, v.get());
   HIP_LAUNCH_CHECK();
-  auto norm_into = [&](const double* x, double* out) {
-    sumsq_kernel<<<nb, kNT, 0, s>>>(n, x, part.get());
+  auto norm_into = [&](double* x, const double* scale, double* out) {
+    sumsq_kernel<<<nb, kNT, 0, s>>>(n, x, scale, part.get());
     HIP_LAUNCH_CHECK();
     norm_finish_kernel<<<1, 64, 0, s>>>(nb, part.get(), out);
     HIP_LAUNCH_CHECK();
   };
-  norm_into(v.get(), nrm.get());
+  norm_into(v.get(), nullptr, nrm.get());
   scale_by_norm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nrm.get(), v.get());
   HIP_LAUNCH_CHECK();
   for (int it = 0; it < iters; ++it) {
     spmv64(A, v.get(), w.get(), s);
-    dinv_scale_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, dinv, w.get());
-    HIP_LAUNCH_CHECK();
-    norm_into(w.get(), nrm.get() + 1 + it);
+    norm_into(w.get(), dinv, nrm.get() + 1 + it);  // w = D^-1 A v and its norm
     scale_by_norm_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, nrm.get() + 1 + it, w.get());
     HIP_LAUNCH_CHECK();
     std::swap(v, w);
