@@ -767,13 +767,16 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
   if (4 * ne >= (int64_t)1 << 31 || nv >= (int64_t)1 << 31)
     fail(FEM_ERR_CAPACITY, "mesh too large for 32-bit element/vertex indices");
 
-  DevBuf<double> xyz_u;
-  xyz_u.upload(xyz, (size_t)(3 * nv), s);
-  DevBuf<int32_t> conn_u;
-  conn_u.upload(conn, (size_t)(npe * ne), s);
+  DevBuf<double> xyz_u((size_t)(3 * nv));
+  DevBuf<int32_t> conn_u((size_t)(npe * ne));
   DevBuf<double> kappa_u(ne);
-  if (kappa) kappa_u.upload(kappa, (size_t)ne, s);
-  else dev_fill_f64(kappa_u.get(), 1.0, (size_t)ne, s);
+  {
+    const HostPiece pc[3] = {{xyz_u.get(), xyz, sizeof(double) * (size_t)(3 * nv)},
+                             {conn_u.get(), conn, sizeof(int32_t) * (size_t)(npe * ne)},
+                             {kappa_u.get(), kappa, kappa ? sizeof(double) * (size_t)ne : 0}};
+    upload_pieces(pc, 3, s);
+  }
+  if (!kappa) dev_fill_f64(kappa_u.get(), 1.0, (size_t)ne, s);
   const int bad = check_mesh_device(npe * ne, conn_u.get(), nv, ne, kappa ? kappa_u.get() : nullptr, 3 * nv,
                                     xyz_u.get(), s);
   if (bad & 1) fail(FEM_ERR_ARG, "element vertex index out of range");

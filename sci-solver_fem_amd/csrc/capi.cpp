@@ -432,8 +432,9 @@ int fem_update_kappa(fem_handle* hh, const double* kappa) {
   return guarded(hh, [&]() {
     Handle& h = *H(hh);
     if (!h.has_mesh || h.matrix_from_file) fail(FEM_ERR_STATE, "operator is not assembled from a mesh");
-    DevBuf<double> ku;
-    ku.upload(kappa, h.ne, h.stream);
+    DevBuf<double> ku(h.ne);
+    const HostPiece pc = {ku.get(), kappa, sizeof(double) * (size_t)h.ne};
+    upload_pieces(&pc, 1, h.stream);
     if (check_mesh_device(0, nullptr, 0, h.ne, ku.get(), 0, nullptr, h.stream) & 2)
       fail(FEM_ERR_NUMERIC, "NaN in element coefficient");
     set_kappa_user(h, ku.get());

@@ -53,6 +53,19 @@ class DevPoolScope {
   void* prev_;
 };
 
+// Host -> device copy of pageable host memory in pieces (destination, source,
+// bytes).  The HIP runtime stages pageable memory through the calling
+// thread, so one thread bounds the rate (29 GB/s at 690 MB); large copies
+// are split over kUploadThreads host threads, each on a stream of its own
+// kept per device (51 GB/s with four).  Orders after the work queued on s
+// and returns when the data is on the device.
+struct HostPiece {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+void upload_pieces(const HostPiece* pieces, int npieces, hipStream_t s);
+
 // Owning device allocation.  Moves only.
 template <typename T>
 class DevBuf {

@@ -1,8 +1,13 @@
-// Setup-phase device allocation pool (see DevPoolScope in common.hpp).
+// Setup-phase device allocation pool (see DevPoolScope in common.hpp) and
+// the threaded host -> device upload.
 
+#include <algorithm>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 
+#include "../../include/fem_mi355x.h"
 #include "common.hpp"
 
 namespace fem {
@@ -59,6 +64,85 @@ DevPoolScope::~DevPoolScope() {
   for (auto& kv : pl->free_) (void)hipFree(kv.second);
   delete pl;
   g_pool = static_cast<Pool*>(prev_);
+}
+
+namespace {
+
+constexpr int kUploadThreads = 4;
+constexpr size_t kUploadSplit = (size_t)8 << 20;  // smaller copies stay one async copy on s
+constexpr int kMaxDevices = 64;
+std::mutex g_up_mu;
+hipStream_t g_up[kMaxDevices][kUploadThreads];  // per device, created on first use, kept
+
+// bytes [a, b) of the pieces taken end to end, copied on st
+hipError_t upload_range(const HostPiece* pc, int np, size_t a, size_t b, hipStream_t st) {
+  size_t base = 0;
+  for (int i = 0; i < np; ++i) {
+    const size_t lo = std::max(a, base), hi = std::min(b, base + pc[i].bytes);
+    if (lo < hi) {
+      const hipError_t e = hipMemcpyAsync(static_cast<char*>(pc[i].dst) + (lo - base),
+                                          static_cast<const char*>(pc[i].src) + (lo - base), hi - lo,
+                                          hipMemcpyHostToDevice, st);
+      if (e != hipSuccess) return e;
+    }
+    base += pc[i].bytes;
+  }
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace
+
+void upload_pieces(const HostPiece* pc, int np, hipStream_t s) {
+  size_t total = 0;
+  for (int i = 0; i < np; ++i) total += pc[i].bytes;
+  // the destinations may be blocks that earlier work on s still uses
+  HIP_CHECK(hipStreamSynchronize(s));
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(FEM_ERR_CAPACITY, "device index beyond the upload stream table");
+  hipStream_t st[kUploadThreads];
+  {
+    // the streams are created, and each takes a first small copy (a prefix
+    // of the first piece, so what lands there is the data itself), on the
+    // first upload of the process, whatever its size: a stream's first copy
+    // costs milliseconds more than the later ones
+    std::lock_guard<std::mutex> lk(g_up_mu);
+    for (int k = 0; k < kUploadThreads; ++k) {
+      if (!g_up[dev][k]) {
+        HIP_CHECK(hipStreamCreateWithFlags(&g_up[dev][k], hipStreamNonBlocking));
+        const HostPiece* first = nullptr;
+        for (int i = 0; i < np && !first; ++i)
+          if (pc[i].bytes) first = &pc[i];
+        if (first) {
+          HIP_CHECK(hipMemcpyAsync(first->dst, first->src, std::min<size_t>(first->bytes, 4096),
+                                   hipMemcpyHostToDevice, g_up[dev][k]));
+          HIP_CHECK(hipStreamSynchronize(g_up[dev][k]));
+        }
+      }
+      st[k] = g_up[dev][k];
+    }
+  }
+  if (total < kUploadSplit) {
+    for (int i = 0; i < np; ++i)
+      if (pc[i].bytes) HIP_CHECK(hipMemcpyAsync(pc[i].dst, pc[i].src, pc[i].bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  hipError_t err[kUploadThreads];
+  std::vector<std::thread> th;
+  for (int k = 0; k < kUploadThreads; ++k) {
+    const size_t a = total * k / kUploadThreads, b = total * (k + 1) / kUploadThreads;
+    auto work = [&err, pc, np, a, b, dev, k, &st]() {
+      hipError_t e = hipSetDevice(dev);  // the current device is per thread
+      err[k] = e == hipSuccess ? upload_range(pc, np, a, b, st[k]) : e;
+    };
+    try {
+      th.emplace_back(work);
+    } catch (...) {
+      work();  // no thread: copy this share here
+    }
+  }
+  for (auto& t : th) t.join();
+  for (int k = 0; k < kUploadThreads; ++k) HIP_CHECK(err[k]);
 }
 
 }  // namespace fem
