@@ -298,6 +298,7 @@ void fem_destroy(fem_handle* hh) {
   hipStream_t s = h->own_stream ? h->stream : nullptr;
   delete h;  // device buffers free themselves
   if (s) (void)hipStreamDestroy(s);
+  pool_flush_reserve();  // the setup and mesh phases' cached free blocks
 }
 
 const char* fem_last_error(const fem_handle* h) {

@@ -17,6 +17,7 @@ namespace {
 struct Pool {
   std::unordered_map<void*, size_t> owned;  // blocks this pool allocated -> size class
   std::multimap<size_t, void*> free_;       // released blocks by size class
+  int device = 0;
 };
 
 thread_local Pool* g_pool = nullptr;
@@ -30,7 +31,35 @@ size_t size_class(size_t bytes) {
   return (bytes + step - 1) / step * step;
 }
 
+// free blocks of ended scopes, per device, by size class
+constexpr size_t kReserveCap = (size_t)8 << 30;
+struct Reserve {
+  std::multimap<size_t, void*> blocks;
+  size_t bytes = 0;
+};
+std::mutex g_res_mu;
+std::unordered_map<int, Reserve> g_res;
+
 }  // namespace
+
+void pool_flush_reserve() {
+  std::lock_guard<std::mutex> lk(g_res_mu);
+  for (auto& dr : g_res) {
+    for (auto& kv : dr.second.blocks) (void)hipFree(kv.second);
+    dr.second.blocks.clear();
+    dr.second.bytes = 0;
+  }
+}
+
+void* dev_malloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    pool_flush_reserve();
+    HIP_CHECK(hipMalloc(&p, bytes));
+  }
+  return p;
+}
 
 void* pool_alloc(size_t bytes) {
   Pool* pl = g_pool;
@@ -43,7 +72,19 @@ void* pool_alloc(size_t bytes) {
     return p;
   }
   void* p = nullptr;
-  HIP_CHECK(hipMalloc(&p, c));
+  {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    auto rt = g_res.find(pl->device);
+    if (rt != g_res.end()) {
+      auto jt = rt->second.blocks.find(c);
+      if (jt != rt->second.blocks.end()) {
+        p = jt->second;
+        rt->second.blocks.erase(jt);
+        rt->second.bytes -= c;
+      }
+    }
+  }
+  if (!p) p = dev_malloc(c);
   pl->owned.emplace(p, c);
   return p;
 }
@@ -57,11 +98,28 @@ bool pool_release(void* p) {
   return true;
 }
 
-DevPoolScope::DevPoolScope() : prev_(g_pool) { g_pool = new Pool(); }
+DevPoolScope::DevPoolScope() : prev_(g_pool) {
+  Pool* pl = new Pool();
+  (void)hipGetDevice(&pl->device);
+  g_pool = pl;
+}
 
 DevPoolScope::~DevPoolScope() {
   Pool* pl = g_pool;
-  for (auto& kv : pl->free_) (void)hipFree(kv.second);
+  if (!pl->free_.empty()) {
+    // queued work may still use the blocks: only an idle device hands them on
+    const bool idle = hipDeviceSynchronize() == hipSuccess;
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    Reserve& r = g_res[pl->device];
+    for (auto& kv : pl->free_) {
+      if (idle && r.bytes + kv.first <= kReserveCap) {
+        r.blocks.emplace(kv.first, kv.second);
+        r.bytes += kv.first;
+      } else {
+        (void)hipFree(kv.second);
+      }
+    }
+  }
   delete pl;
   g_pool = static_cast<Pool*>(prev_);
 }
