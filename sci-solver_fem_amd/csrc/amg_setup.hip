@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kProNT) prolongator_kernel(int64_t n, const in
                                                            const int32_t* __restrict__ p_ci, double* __restrict__ p_v) {
   __shared__ int s_c[kProRowCap * kProNT];
   __shared__ double s_v[kProRowCap * kProNT];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // XCD-contiguous rows
   if (i >= n) return;
   const int o = p_rp[i], len = p_rp[i + 1] - o;
   const double d = (dinv[i] != 0.0) ? 1.0 / dinv[i] : 0.0;

@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kRowBlock = 64;   // threads (rows) per block for row-owner kernels
 constexpr int kRowCap = 64;     // LDS row capacity; longer rows use the global path
+constexpr int kIncChunk = 8;    // incidences whose loads a row-owner thread issues together
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
   v &= 0x1fffffull;
@@ -170,7 +171,8 @@ __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
     const int4* __restrict__ conn, int32_t* __restrict__ cnt, const int32_t* __restrict__ rp,
     int32_t* __restrict__ ci, const int32_t* __restrict__ sptr = nullptr, int2* __restrict__ inc = nullptr) {
   __shared__ int lds[kRowBlock * kRowCap];
-  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous rows: neighbouring rows gather the same elements through one L2
+  int64_t row = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (row >= nv) return;
   if (kFill && cnt[row] < 0) return;
   // lane-interleaved rows (entry t of lane j at t * kRowBlock + j): the
@@ -178,10 +180,17 @@ __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
   const LdsRow buf{lds + threadIdx.x};
   int len = 0;
   len = sorted_insert(buf, len, kRowCap, (int)row);
-  for (int p = v2e_ptr[row]; p < v2e_ptr[row + 1] && len >= 0; ++p) {
-    int4 c = conn[v2e[p] >> 2];
-    int v[4] = {c.x, c.y, c.z, c.w};
-    for (int a = 0; a < npe && len >= 0; ++a) len = sorted_insert(buf, len, kRowCap, v[a]);
+  const int pb = v2e_ptr[row], pe = v2e_ptr[row + 1];
+  // the element codes and vertices of kIncChunk incidences are loaded
+  // together (independent loads), then inserted in incidence order
+  for (int p0 = pb; p0 < pe && len >= 0; p0 += kIncChunk) {
+    int4 cc[kIncChunk];
+#pragma unroll
+    for (int u = 0; u < kIncChunk; ++u) cc[u] = p0 + u < pe ? conn[v2e[p0 + u] >> 2] : make_int4(0, 0, 0, 0);
+    for (int u = 0; u < kIncChunk && p0 + u < pe && len >= 0; ++u) {
+      const int v[4] = {cc[u].x, cc[u].y, cc[u].z, cc[u].w};
+      for (int a = 0; a < npe && len >= 0; ++a) len = sorted_insert(buf, len, kRowCap, v[a]);
+    }
   }
   if (!kFill) {
     cnt[row] = len;  // -1 marks overflow
@@ -193,20 +202,26 @@ __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
       const int lane = (int)(row & 63);
       const int k0 = sptr[sl], kn = sptr[sl + 1] - k0;
       // kn == 0: the slice sums in place and has no incidences
-      for (int p = v2e_ptr[row], k = 0; kn > 0 && p < v2e_ptr[row + 1]; ++p, ++k) {
-        const int code = v2e[p];
-        const int4 c = conn[code >> 2];
-        const int v[4] = {c.x, c.y, c.z, c.w};
-        unsigned slots = 0;
-        for (int b = 0; b < npe; ++b) {
-          int lo = 0, hi = len;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (buf[mid] < v[b]) lo = mid + 1; else hi = mid;
+      for (int p0 = pb; kn > 0 && p0 < pe; p0 += kIncChunk) {
+        int code[kIncChunk];
+        int4 cc[kIncChunk];
+#pragma unroll
+        for (int u = 0; u < kIncChunk; ++u) code[u] = p0 + u < pe ? v2e[p0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < kIncChunk; ++u) cc[u] = p0 + u < pe ? conn[code[u] >> 2] : make_int4(0, 0, 0, 0);
+        for (int u = 0; u < kIncChunk && p0 + u < pe; ++u) {
+          const int v[4] = {cc[u].x, cc[u].y, cc[u].z, cc[u].w};
+          unsigned slots = 0;
+          for (int b = 0; b < npe; ++b) {
+            int lo = 0, hi = len;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (buf[mid] < v[b]) lo = mid + 1; else hi = mid;
+            }
+            slots |= (unsigned)lo << (8 * b);
           }
-          slots |= (unsigned)lo << (8 * b);
+          inc[((int64_t)(k0 + (p0 + u - pb)) << 6) + lane] = make_int2(code[u], (int)slots);
         }
-        inc[((int64_t)(k0 + k) << 6) + lane] = make_int2(code, (int)slots);
       }
     }
   }

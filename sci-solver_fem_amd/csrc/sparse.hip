@@ -15,6 +15,7 @@
 #include <climits>
 
 #include "fem_internal.hpp"
+#include "kernels.hpp"
 #include "half.hpp"
 
 #include <algorithm>
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(kSymNT) sym_group_stash_kernel(int64_t nrows, 
   __shared__ int s_cnt[RPB];
   __shared__ int s_over[RPB];
   const int g = (int)threadIdx.x / G, lane = (int)threadIdx.x % G;
-  const int64_t i = (int64_t)blockIdx.x * RPB + g;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * RPB + g;  // XCD-contiguous rows
   const bool live = i < nrows;
   int* tab = s_tab + g * S;
   for (int t = lane; t < CAP; t += G) tab[t] = -1;
@@ -334,7 +335,8 @@ __global__ void __launch_bounds__(kNumNT) num_group_kernel(int64_t nrows, const 
   __shared__ int s_cols[RPB * S];
   __shared__ double s_acc[RPB * S];
   const int g = (int)threadIdx.x / G, lane = (int)threadIdx.x % G;
-  const int64_t i = (int64_t)blockIdx.x * RPB + g;
+  // XCD-contiguous rows: neighbouring rows gather the same R rows through one L2
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * RPB + g;
   const bool live = i < nrows;
   const int o = live ? c_rp[i] : 0;
   const int len = live ? c_rp[i + 1] - o : 0;
