@@ -117,13 +117,16 @@ int fem_create_from_files(const char* base, int is_tet, int verbose, fem_handle*
  * xyz is nv x 3 (z ignored in 2D).  kappa: per-element conductivity or NULL
  * (all ones).  Operator K = sum_e kappa_e S_e + mass_coeff * M.
  * flags bit 0: 2D mass with the signed Jacobian
- * (src/core/cuda/perform_element_loop_2D.cuh:92). */
+ * (src/core/cuda/perform_element_loop_2D.cuh:92).  The host arrays are
+ * copied by four library threads (one HIP stream each, kept per device). */
 int fem_create_from_arrays(int dim, int64_t nv, const double* xyz, int64_t ne,
                            const int32_t* conn, const double* kappa,
                            double mass_coeff, int flags, int verbose,
                            fem_handle** out);
 
-/* FEMSolver::~FEMSolver, src/FEMSolver.cu:46-51. */
+/* FEMSolver::~FEMSolver, src/FEMSolver.cu:46-51.  Also frees the reserve of
+ * temporary device buffers that mesh builds and setups keep for reuse
+ * (at most 8 GiB per device). */
 void fem_destroy(fem_handle* h);
 
 /* Message of the last failure on h (or of the last failed create if h is NULL). */
