@@ -124,10 +124,16 @@ int fem_create_from_arrays(int dim, int64_t nv, const double* xyz, int64_t ne,
                            double mass_coeff, int flags, int verbose,
                            fem_handle** out);
 
-/* FEMSolver::~FEMSolver, src/FEMSolver.cu:46-51.  Also frees the reserve of
- * temporary device buffers that mesh builds and setups keep for reuse
- * (at most 8 GiB per device). */
+/* FEMSolver::~FEMSolver, src/FEMSolver.cu:46-51. */
 void fem_destroy(fem_handle* h);
+
+/* No counterpart in the reference.  Frees the library's reserve of device
+ * blocks: temporaries that mesh builds and AMG setups release are kept (at
+ * most 8 GiB per device) for later builds and setups to reuse, instead of a
+ * hipFree each.  The reserve is also freed whenever a library allocation
+ * fails.  For callers that need the memory for their own allocations; must
+ * not overlap with a fem_create_* or fem_setup call. */
+void fem_release_cached_memory(void);
 
 /* Message of the last failure on h (or of the last failed create if h is NULL). */
 const char* fem_last_error(const fem_handle* h);

@@ -21,7 +21,7 @@ from ._lib import FemParams, FemSolveInfo
 from . import meshgen  # noqa: F401  (re-exported)
 
 __all__ = ["FEMSolver", "FemError", "NotConverged", "fem_solve", "FEMSolveFunction", "meshgen",
-           "read_matlab_array", "write_matlab_array", "read_tetgen", "read_trimesh",
+           "release_cached_memory", "read_matlab_array", "write_matlab_array", "read_tetgen", "read_trimesh",
            "read_matlab_sparse_coo", "device_aggregate", "lib"]
 
 
@@ -407,6 +407,12 @@ class FEMSolver:
 
 
 # ---- module-level IO helpers (no GPU needed) ------------------------------
+
+def release_cached_memory() -> None:
+    """Free the library's reserve of device blocks (temporaries of mesh builds
+    and setups kept for reuse, at most 8 GiB per device)."""
+    lib().fem_release_cached_memory()
+
 
 def read_matlab_array(filename: str) -> np.ndarray:
     L = lib()

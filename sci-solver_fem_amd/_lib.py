@@ -82,6 +82,7 @@ SIGNATURES = {
     "fem_create_from_arrays": (C.c_int, [C.c_int, _I64, _P, _I64, _P, _P, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(_P)]),
     "fem_destroy": (None, [_P]),
+    "fem_release_cached_memory": (None, []),
     "fem_last_error": (C.c_char_p, [_P]),
     "fem_set_params": (C.c_int, [_P, C.POINTER(FemParams)]),
     "fem_get_params": (C.c_int, [_P, C.POINTER(FemParams)]),

@@ -287,6 +287,8 @@ int fem_create_from_files(const char* base, int is_tet, int verbose, fem_handle*
                                 m.dim == 2 ? 1 : 0, verbose, out);
 }
 
+void fem_release_cached_memory(void) { pool_flush_reserve(); }
+
 void fem_destroy(fem_handle* hh) {
   Handle* h = H(hh);
   if (!h) return;
@@ -298,7 +300,6 @@ void fem_destroy(fem_handle* hh) {
   hipStream_t s = h->own_stream ? h->stream : nullptr;
   delete h;  // device buffers free themselves
   if (s) (void)hipStreamDestroy(s);
-  pool_flush_reserve();  // the setup and mesh phases' cached free blocks
 }
 
 const char* fem_last_error(const fem_handle* h) {

@@ -39,8 +39,9 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 // of such a phase runs on one stream, which orders a block's reuse after its
 // earlier uses.  When the scope ends its free blocks go, after a device
 // synchronization, to a per-device reserve of at most 8 GiB that later scopes
-// draw from (hipFree of each cost 0.25-0.65 ms); the reserve is freed by
-// pool_flush_reserve (fem_destroy calls it) and whenever an allocation fails.
+// draw from (hipFree of each cost 0.25-0.65 ms, and large frees slow the
+// allocations after them); the reserve is freed by pool_flush_reserve
+// (fem_release_cached_memory) and whenever an allocation fails.
 // Blocks still held when the scope ends (buffers the phase keeps) are freed
 // by hipFree whenever their owner releases them.
 void* pool_alloc(size_t bytes);   // null when no scope is active on this thread

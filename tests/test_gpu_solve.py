@@ -292,3 +292,28 @@ def test_batched_widths(fem, gpu, oracle, meshgen, k, smoother, precision):
         assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1e-9
         assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) <= 1e-7
     s.close()
+
+
+def test_handles_reuse_the_device_reserve(fem, gpu, oracle, meshgen):
+    """Three handles in turn on a 41^3 mesh (10.8 MB of host arrays: the
+    threaded upload): the second draws its temporaries from the blocks the
+    first's mesh build and setup released, the third runs after
+    release_cached_memory() freed them.  Stale contents of reused blocks must
+    not matter: the three solutions are bitwise equal and solve the oracle's
+    system."""
+    report("[gpu] device reserve: three handles, release in between")
+    xyz, conn, kappa, A, b = _problem(meshgen, oracle, 41, 1.0, 1e-2, seed=6)
+    assert xyz.nbytes + conn.nbytes + kappa.nbytes >= 8 << 20
+    xs = []
+    for k in range(3):
+        if k == 2:
+            fem.release_cached_memory()
+        s = fem.FEMSolver.from_arrays(3, xyz, conn, kappa=kappa, mass_coeff=1e-2)
+        s.set_params(solverType=1, tolerance=1e-10, maxIters=300, precision=1, smootherType=1,
+                     preInnerIters=2, postInnerIters=2, topSize=64)
+        x, info = s.solve(b)
+        assert info["converged"] == 1
+        xs.append(x)
+        s.close()
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+    assert np.linalg.norm(b - A @ xs[0]) / np.linalg.norm(b) <= 1e-9
