@@ -60,8 +60,10 @@ typedef struct fem_params {
   int aggregatorType;     /* aggregatorType_     0  randomized MIS-2; 3 AggMIS:
                              MIS, aggregate-to-nearest, merge-split to sizes
                              [p/1000%1000, p%1000] of randMisParameters p
-                             (MIS depth p/10^7%10, 0 = 2); 1, 2, 4, 5: not
-                             built (FEM_ERR_ARG)                              */
+                             (MIS depth p/10^7%10, 0 = 2); 4 / 5: host
+                             flood-fill MIS of depth p/10^7%10, nearest-root
+                             aggregation, merge-split for 4 only; 1, 2 (METIS):
+                             not built (FEM_ERR_ARG)                          */
   int convergeType;       /* convergeType_       0  accepted, no effect       */
   double tolerance;       /* tolerance_          1e-6 relative residual       */
   int cycleType;          /* cycleType_          0  V; 1 W (coarse correction:
@@ -283,6 +285,19 @@ int fem_device_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj,
  * rewrites aggregate and returns the new count in *n_agg_out. */
 int fem_condition_aggregates(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* aggregate,
                              int64_t n_agg, int min_size, int max_size, int64_t* n_agg_out);
+
+/* aggregatorType 4 / 5, host diagnostic: the flood-fill distance-`depth` MIS
+ * (AggMIS::MIS::FloodFillMIS, src/core/aggmis/cuda/AggMIS_MIS_CPU.cu:25-140)
+ * of a symmetric graph (CSR without the diagonal) from the start node the
+ * setup derives from `seed` (returned in *starter_out when not NULL; the
+ * reference starts at rand() % n after srand(time(NULL))), and every node
+ * aggregated to the nearest root (AggMIS::Aggregation::AggregateToNearest,
+ * AggMIS_Aggregation_CPU.cu:4-92).  roots_out[i] = 1 for a root, 0 else;
+ * aggregate ids are dense, numbered in root index order.  Type 4 then runs
+ * fem_condition_aggregates; type 5 uses the aggregation as it is. */
+int fem_flood_fill_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth,
+                             unsigned long long seed, int64_t* starter_out, int32_t* roots_out,
+                             int32_t* aggregate_out, int64_t* n_agg_out);
 
 /* ---- profiling -------------------------------------------------------------*/
 /* Time each launch of the named kernel family with HIP events on the handle's

@@ -47,6 +47,11 @@ extern "C" {
 #endif
 int64_t orc_condition(int64_t n, const int32_t* ap, const int32_t* aj, int32_t* agg, int64_t nagg, int min_size,
                       int max_size);
+/* aggregatorType 4 / 5 (fem_oracle_aggmis.c): flood-fill distance-k MIS from `starter`
+ * (AggMIS_MIS_CPU.cu:25-140) and aggregation to the nearest root (AggMIS_Aggregation_CPU.cu:4-92) */
+void orc_flood_fill_mis(int64_t n, const int32_t* ap, const int32_t* aj, int k, int64_t starter, int32_t* mis);
+int64_t orc_aggregate_to_nearest(int64_t n, const int32_t* ap, const int32_t* aj, const int32_t* roots,
+                                 int32_t* agg);
 #ifdef __cplusplus
 }
 #endif

@@ -458,3 +458,189 @@ int64_t orc_condition(int64_t n, const int32_t* ap, const int32_t* aj, int32_t* 
   free(c.gj);
   return out;
 }
+
+/* ---- aggregatorType 4 / 5: flood-fill MIS and aggregation to the nearest root ----
+ *
+ * Reference: AggMIS::MIS::FloodFillMIS, src/core/aggmis/cuda/AggMIS_MIS_CPU.cu:25-140, and
+ * AggMIS::Aggregation::AggregateToNearest, src/core/aggmis/cuda/AggMIS_Aggregation_CPU.cu:4-92,
+ * called by NewMIS_CPU (type 4, then the merge-split conditioner) and LightMIS_CPU (type 5),
+ * src/core/cuda/ComputePermutationMethods.cu:566-860.  The reference starts at rand() % n after
+ * srand(time(NULL)); here the start node is an argument.  Ties in the fringe's visit counts,
+ * which the reference leaves to std::priority_queue's heap layout, go in push order. */
+
+typedef struct {
+  int32_t node, visits;
+  int64_t seq;
+} orc_fringe_entry;
+
+/* a before b in the fringe: more visits, then earlier push */
+static int orc_fringe_first(const orc_fringe_entry* a, const orc_fringe_entry* b) {
+  return a->visits != b->visits ? a->visits > b->visits : a->seq < b->seq;
+}
+
+typedef struct {
+  orc_fringe_entry* e;
+  int64_t n, cap;
+} orc_heap;
+
+static void orc_heap_push(orc_heap* h, orc_fringe_entry x) {
+  if (h->n == h->cap) {
+    h->cap = h->cap ? 2 * h->cap : 1024;
+    h->e = (orc_fringe_entry*)realloc(h->e, (size_t)h->cap * sizeof(orc_fringe_entry));
+  }
+  int64_t i = h->n++;
+  while (i > 0) {
+    int64_t p = (i - 1) / 2;
+    if (!orc_fringe_first(&x, &h->e[p])) break;
+    h->e[i] = h->e[p];
+    i = p;
+  }
+  h->e[i] = x;
+}
+
+static orc_fringe_entry orc_heap_pop(orc_heap* h) {
+  orc_fringe_entry top = h->e[0], last = h->e[--h->n];
+  int64_t i = 0;
+  for (;;) {
+    int64_t c = 2 * i + 1;
+    if (c >= h->n) break;
+    if (c + 1 < h->n && orc_fringe_first(&h->e[c + 1], &h->e[c])) ++c;
+    if (!orc_fringe_first(&h->e[c], &last)) break;
+    h->e[i] = h->e[c];
+    i = c;
+  }
+  if (h->n > 0) h->e[i] = last;
+  return top;
+}
+
+typedef struct {
+  int32_t* v;
+  int64_t head, tail, cap;
+} orc_fifo;
+
+static void orc_fifo_push(orc_fifo* q, int32_t x) {
+  if (q->tail == q->cap) {
+    if (q->head > 0) { /* slide the live part to the front */
+      memmove(q->v, q->v + q->head, (size_t)(q->tail - q->head) * sizeof(int32_t));
+      q->tail -= q->head;
+      q->head = 0;
+    }
+    if (q->tail == q->cap) {
+      q->cap = q->cap ? 2 * q->cap : 1024;
+      q->v = (int32_t*)realloc(q->v, (size_t)q->cap * sizeof(int32_t));
+    }
+  }
+  q->v[q->tail++] = x;
+}
+
+/* mis[i] = 1 for a root, 0 otherwise; every node within distance k of a root, no two roots
+ * within distance k of each other. */
+void orc_flood_fill_mis(int64_t n, const int32_t* ap, const int32_t* aj, int k, int64_t starter, int32_t* mis) {
+  int32_t* visits = (int32_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+  int32_t* dist = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+  for (int64_t i = 0; i < n; ++i) {
+    mis[i] = -1;
+    dist[i] = 1000;
+  }
+  orc_heap fringe = {0};
+  orc_fifo frontier = {0};
+  int64_t seq = 0, start = starter;
+  while (n > 0) {
+    if (mis[start] == -1) {
+      orc_fringe_entry s0 = {(int32_t)start, 1, seq++};
+      orc_heap_push(&fringe, s0);
+    }
+    for (;;) {
+      int32_t root = -1;
+      while (fringe.n > 0) {
+        orc_fringe_entry c = orc_heap_pop(&fringe);
+        if (dist[c.node] > k) {
+          root = c.node;
+          break;
+        }
+      }
+      if (root < 0) break;
+      mis[root] = 1;
+      dist[root] = 0;
+      for (int32_t q = ap[root]; q < ap[root + 1]; ++q)
+        if (dist[aj[q]] > 1) {
+          dist[aj[q]] = 1;
+          orc_fifo_push(&frontier, aj[q]);
+        }
+      while (frontier.head < frontier.tail) {
+        const int32_t u = frontier.v[frontier.head++];
+        const int32_t d = dist[u];
+        mis[u] = 0;
+        if (d < k) {
+          for (int32_t q = ap[u]; q < ap[u + 1]; ++q)
+            if (dist[aj[q]] > d + 1) {
+              dist[aj[q]] = d + 1;
+              orc_fifo_push(&frontier, aj[q]);
+            }
+        }
+        if (d == k) {
+          for (int32_t q = ap[u]; q < ap[u + 1]; ++q)
+            if (dist[aj[q]] >= d + 1) {
+              dist[aj[q]] = d + 1;
+              orc_fringe_entry f = {aj[q], ++visits[aj[q]], seq++};
+              orc_heap_push(&fringe, f);
+            }
+        }
+      }
+    }
+    int64_t next = -1;
+    for (int64_t i = 0; i < n; ++i)
+      if (mis[i] == -1) {
+        next = i;
+        break;
+      }
+    if (next < 0) break;
+    start = next;
+  }
+  free(fringe.e);
+  free(frontier.v);
+  free(visits);
+  free(dist);
+}
+
+/* agg[i] = aggregate of node i (roots numbered in index order); returns the aggregate count. */
+int64_t orc_aggregate_to_nearest(int64_t n, const int32_t* ap, const int32_t* aj, const int32_t* roots,
+                                 int32_t* agg) {
+  int32_t* rd = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+  orc_fifo todo = {0};
+  int32_t next = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (roots[i] == 1) {
+      rd[i] = 0;
+      agg[i] = next++;
+      for (int32_t q = ap[i]; q < ap[i + 1]; ++q) orc_fifo_push(&todo, aj[q]);
+    } else {
+      rd[i] = -1;
+      agg[i] = -1;
+    }
+  }
+  while (todo.head < todo.tail) {
+    const int32_t u = todo.v[todo.head++];
+    if (agg[u] != -1 && rd[u] != -1) continue;
+    int32_t best = -1, bd = -1;
+    for (int32_t q = ap[u]; q < ap[u + 1]; ++q) {
+      const int32_t v = aj[q], d = rd[v];
+      if (d >= 0) {
+        if (bd == -1) {
+          bd = d;
+          best = agg[v];
+        } else if (d < bd) {
+          best = agg[v];
+          bd = d;
+        }
+      } else {
+        orc_fifo_push(&todo, v);
+      }
+    }
+    agg[u] = best;
+    rd[u] = bd + 1;
+  }
+  free(todo.v);
+  free(rd);
+  return next;
+}

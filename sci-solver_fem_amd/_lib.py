@@ -111,6 +111,7 @@ SIGNATURES = {
     "fem_get_permutation": (C.c_int, [_P, _P]),
     "fem_device_aggregate": (C.c_int, [_I64, _P, _P, C.c_int, C.c_int, C.c_int, C.c_ulonglong, _P, _PI32]),
     "fem_condition_aggregates": (C.c_int, [_I64, _P, _P, _P, _I64, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
+    "fem_flood_fill_aggregate": (C.c_int, [_I64, _P, _P, C.c_int, C.c_ulonglong, _P, _P, _P, _P]),
     "fem_profile_enable": (C.c_int, [_P, C.c_int]),
     "fem_profile_get": (C.c_int, [_P, C.c_char_p, _PI64, _PD, _PD]),
     "fem_read_tetgen": (C.c_int, [C.c_char_p, _PI64, C.POINTER(_PD), _PI64, C.POINTER(_PI32),

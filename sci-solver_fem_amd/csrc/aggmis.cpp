@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <exception>
+#include <queue>
 #include <thread>
 #include <vector>
 
@@ -450,6 +451,129 @@ int64_t condition_aggregates(int64_t n, const int32_t* rp, const int32_t* ci, st
   if (n == 0 || nagg == 0) return nagg;
   MergeSplit ms(n, rp, ci, agg, nagg, min_size, max_size);
   return ms.condition();
+}
+
+// ---- aggregatorType 4 and 5: flood-fill MIS and aggregation to the nearest root (host) ----
+
+int64_t flood_fill_starter(int64_t n, unsigned long long seed) {
+  // the reference starts at rand() % n after srand(time(NULL)); here a
+  // splitmix64 step of the seed, so the result is reproducible
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return n > 0 ? (int64_t)(z % (uint64_t)n) : 0;
+}
+
+// Distance-k maximal independent set by flood fill (AggMIS::MIS::FloodFillMIS,
+// src/core/aggmis/cuda/AggMIS_MIS_CPU.cu:25-140).  A root is taken from the
+// fringe (nodes just beyond distance k of the roots so far, most often
+// reached first), its k-neighbourhood is marked out breadth-first, and nodes
+// at distance k + 1 join the fringe; when the fringe runs dry the lowest
+// undecided node starts the next component.  The reference leaves ties in
+// the fringe's visit counts to its heap layout; here they go in push order.
+std::vector<int32_t> flood_fill_mis(int64_t n, const int32_t* rp, const int32_t* ci, int k, int64_t starter) {
+  std::vector<int32_t> mis((size_t)n, -1), visits((size_t)n, 0), dist((size_t)n, 1000);
+  if (n == 0) return mis;
+  struct Entry {
+    int32_t node, visits;
+    int64_t seq;
+  };
+  struct Lower {  // the top is the most visits, then the earliest push
+    bool operator()(const Entry& a, const Entry& b) const {
+      return a.visits != b.visits ? a.visits < b.visits : a.seq > b.seq;
+    }
+  };
+  std::priority_queue<Entry, std::vector<Entry>, Lower> fringe;
+  std::queue<int32_t> frontier;
+  int64_t seq = 0, cursor = 0, start = starter;
+  while (true) {
+    if (mis[(size_t)start] == -1) fringe.push(Entry{(int32_t)start, 1, seq++});
+    while (true) {
+      int32_t root = -1;
+      while (!fringe.empty()) {
+        const Entry c = fringe.top();
+        fringe.pop();
+        if (dist[(size_t)c.node] > k) {
+          root = c.node;
+          break;
+        }
+      }
+      if (root < 0) break;
+      mis[(size_t)root] = 1;
+      dist[(size_t)root] = 0;
+      for (int32_t q = rp[root]; q < rp[root + 1]; ++q) {
+        const int32_t v = ci[q];
+        if (dist[(size_t)v] > 1) {
+          dist[(size_t)v] = 1;
+          frontier.push(v);
+        }
+      }
+      while (!frontier.empty()) {
+        const int32_t u = frontier.front();
+        frontier.pop();
+        const int32_t d = dist[(size_t)u];
+        mis[(size_t)u] = 0;
+        for (int32_t q = rp[u]; q < rp[u + 1]; ++q) {
+          const int32_t v = ci[q];
+          if (d < k && dist[(size_t)v] > d + 1) {
+            dist[(size_t)v] = d + 1;
+            frontier.push(v);
+          } else if (d == k && dist[(size_t)v] >= d + 1) {
+            dist[(size_t)v] = d + 1;
+            fringe.push(Entry{v, ++visits[(size_t)v], seq++});
+          }
+        }
+      }
+    }
+    // decided nodes stay decided, so the lowest undecided one is found by a
+    // cursor that only moves forward
+    while (cursor < n && mis[(size_t)cursor] != -1) ++cursor;
+    if (cursor == n) break;
+    start = cursor;
+  }
+  return mis;
+}
+
+// Every node to an aggregate of the nearest root (AggMIS::Aggregation::
+// AggregateToNearest, src/core/aggmis/cuda/AggMIS_Aggregation_CPU.cu:4-92):
+// roots are numbered in index order and queue their neighbours; a queued
+// node still unassigned joins the assigned neighbour of least root distance
+// (the first in adjacency order among equals) and queues its unassigned
+// neighbours.  Returns the number of aggregates.
+int64_t aggregate_to_nearest(int64_t n, const int32_t* rp, const int32_t* ci, const std::vector<int32_t>& roots,
+                             std::vector<int32_t>& agg) {
+  agg.assign((size_t)n, -1);
+  std::vector<int32_t> rd((size_t)n, -1);
+  std::queue<int32_t> todo;
+  int32_t next = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (roots[(size_t)i] == 1) {
+      rd[(size_t)i] = 0;
+      agg[(size_t)i] = next++;
+      for (int32_t q = rp[i]; q < rp[i + 1]; ++q) todo.push(ci[q]);
+    }
+  while (!todo.empty()) {
+    const int32_t u = todo.front();
+    todo.pop();
+    if (agg[(size_t)u] != -1 && rd[(size_t)u] != -1) continue;
+    int32_t best_agg = -1, best_d = -1;
+    for (int32_t q = rp[u]; q < rp[u + 1]; ++q) {
+      const int32_t v = ci[q];
+      const int32_t d = rd[(size_t)v];
+      if (d >= 0) {
+        if (best_d == -1 || d < best_d) {
+          best_d = d;
+          best_agg = agg[(size_t)v];
+        }
+      } else {
+        todo.push(v);
+      }
+    }
+    agg[(size_t)u] = best_agg;
+    rd[(size_t)u] = best_d + 1;
+  }
+  return next;
 }
 
 }  // namespace fem

@@ -991,9 +991,15 @@ void build_hierarchy(Handle& h) {
   // conditioner.  Depth 0 (the default's digit) is taken as 2: the
   // reference's depth-0 MIS marks no root and its aggregate-to-nearest loop
   // never ends.
+  // aggregatorType 4 and 5 (NewMIS_CPU, LightMIS_CPU, ComputePermutationMethods.cu:566-775 and
+  // :776-860): the same reading, a flood-fill MIS of depth p / 10^7 % 10 on the
+  // host (0 is a plain MIS there), aggregation to the nearest root, and for
+  // type 4 the conditioner (type 5 builds one and does not run it).
   const bool aggmis = h.prm.aggregatorType == 3;
+  const bool flood = h.prm.aggregatorType == 4 || h.prm.aggregatorType == 5;
   const int agg_lo = (h.prm.randMisParameters / 1000) % 1000, agg_hi = h.prm.randMisParameters % 1000;
-  const int agg_depth = (h.prm.randMisParameters / 10000000) % 10 ? (h.prm.randMisParameters / 10000000) % 10 : 2;
+  const int flood_depth = (h.prm.randMisParameters / 10000000) % 10;
+  const int agg_depth = flood_depth ? flood_depth : 2;
 
   // level-0 graph: pattern of A without the diagonal
   Csr G;
@@ -1036,6 +1042,18 @@ void build_hierarchy(Handle& h) {
       L.agg.resize(n);
       unsigned long long seed = h.prm.seed + 0x9E37ull * (unsigned long long)l;
       auto aggregate = [&]() {
+        if (flood) {
+          std::vector<int32_t> grp(n + 1), gci(G.nnz > 0 ? G.nnz : 1), hagg;
+          G.rp.download(grp.data(), n + 1, s);
+          if (G.nnz > 0) G.ci.download(gci.data(), G.nnz, s);
+          HIP_CHECK(hipStreamSynchronize(s));
+          const std::vector<int32_t> roots =
+              flood_fill_mis(n, grp.data(), gci.data(), flood_depth, flood_fill_starter(n, seed));
+          nagg = aggregate_to_nearest(n, grp.data(), gci.data(), roots, hagg);
+          if (h.prm.aggregatorType == 4) nagg = condition_aggregates(n, grp.data(), gci.data(), hagg, nagg, agg_lo, agg_hi);
+          L.agg.upload(hagg.data(), n, s);
+          return;
+        }
         if (!aggmis) {
           aggregate_graph(n, G.rp.get(), G.ci.get(), fine_depth, min_size, h.prm.aggregationRule, seed, L.agg.get(),
                           &nagg, s);

@@ -317,12 +317,13 @@ int fem_set_params(fem_handle* hh, const fem_params* p) {
         p->strengthThreshold >= 1.0 || p->launchMode < 0 || p->launchMode > 1 || p->cycleType < 0 ||
         p->cycleType > 3)
       fail(FEM_ERR_ARG, "invalid solver parameter");
-    if (p->aggregatorType != 0 && p->aggregatorType != 3)
-      fail(FEM_ERR_ARG, "aggregatorType: 0 (randomized MIS) and 3 (AggMIS merge-split) are implemented");
-    if (p->aggregatorType == 3) {
+    if (p->aggregatorType != 0 && (p->aggregatorType < 3 || p->aggregatorType > 5))
+      fail(FEM_ERR_ARG, "aggregatorType: 0 (randomized MIS), 3 (AggMIS merge-split), 4 and 5 (flood-fill MIS, "
+                        "with and without merge-split) are implemented; 1 and 2 need METIS");
+    if (p->aggregatorType >= 3) {
       const int lo = (p->randMisParameters / 1000) % 1000, hi = p->randMisParameters % 1000;
       if (p->randMisParameters < 0 || lo < 1 || hi < lo)
-        fail(FEM_ERR_ARG, "aggregatorType 3 reads aggregate sizes [p / 1000 % 1000, p % 1000] from "
+        fail(FEM_ERR_ARG, "aggregatorType 3-5 read aggregate sizes [p / 1000 % 1000, p % 1000] from "
                           "randMisParameters; they must satisfy 1 <= min <= max");
     }
     if (params_shape_changed(h.prm, *p)) h.H.built = false;
@@ -719,6 +720,28 @@ int fem_condition_aggregates(int64_t n, const int32_t* adj_ptr, const int32_t* a
       if (!used[(size_t)a]) fail(FEM_ERR_ARG, "aggregate ids must be dense (every id in use)");
     *n_agg_out = condition_aggregates(n, adj_ptr, adj, agg, n_agg, min_size, max_size);
     std::copy(agg.begin(), agg.end(), aggregate);
+    return (int)FEM_OK;
+  });
+}
+
+int fem_flood_fill_aggregate(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth,
+                             unsigned long long seed, int64_t* starter_out, int32_t* roots_out,
+                             int32_t* aggregate_out, int64_t* n_agg_out) {
+  if (n < 0 || !adj_ptr || depth < 0 || !roots_out || !aggregate_out || !n_agg_out) return FEM_ERR_ARG;
+  return guarded(nullptr, [&]() {
+    const int64_t m = adj_ptr[n];
+    if (m > 0 && !adj) fail(FEM_ERR_ARG, "adjacency missing");
+    for (int64_t i = 0; i < n; ++i)
+      if (adj_ptr[i] > adj_ptr[i + 1]) fail(FEM_ERR_ARG, "row offsets must not decrease");
+    for (int64_t t = 0; t < m; ++t)
+      if (adj[t] < 0 || adj[t] >= n) fail(FEM_ERR_ARG, "adjacency index out of range");
+    const int64_t start = flood_fill_starter(n, seed);
+    if (starter_out) *starter_out = start;
+    const std::vector<int32_t> roots = flood_fill_mis(n, adj_ptr, adj, depth, start);
+    std::vector<int32_t> agg;
+    *n_agg_out = aggregate_to_nearest(n, adj_ptr, adj, roots, agg);
+    std::copy(roots.begin(), roots.end(), roots_out);
+    std::copy(agg.begin(), agg.end(), aggregate_out);
     return (int)FEM_OK;
   });
 }

@@ -245,6 +245,12 @@ void build_hierarchy(Handle& h);
 // max_size] (host; aggmis.cpp); rewrites agg, returns the aggregate count
 int64_t condition_aggregates(int64_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& agg,
                              int64_t nagg, int min_size, int max_size);
+// aggregatorType 4 / 5 (host; aggmis.cpp): flood-fill distance-k MIS from a
+// start node derived from the seed, aggregation to the nearest root
+int64_t flood_fill_starter(int64_t n, unsigned long long seed);
+std::vector<int32_t> flood_fill_mis(int64_t n, const int32_t* rp, const int32_t* ci, int k, int64_t starter);
+int64_t aggregate_to_nearest(int64_t n, const int32_t* rp, const int32_t* ci, const std::vector<int32_t>& roots,
+                             std::vector<int32_t>& agg);
 void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int depth, int min_size,
                      int rule, unsigned long long seed, int32_t* agg_out, int64_t* n_agg,
                      hipStream_t s);

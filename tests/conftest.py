@@ -79,6 +79,9 @@ class Oracle:
         lib.orc_free.argtypes = [V]
         lib.orc_condition.restype = C.c_int64
         lib.orc_condition.argtypes = [C.c_int64, V, V, V, C.c_int64, C.c_int, C.c_int]
+        lib.orc_flood_fill_mis.argtypes = [C.c_int64, V, V, C.c_int, C.c_int64, V]
+        lib.orc_aggregate_to_nearest.restype = C.c_int64
+        lib.orc_aggregate_to_nearest.argtypes = [C.c_int64, V, V, V, V]
         lib.orc_amg_pcg.restype = C.c_int
         lib.orc_amg_pcg.argtypes = [C.c_int64, V, V, V, V, V, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, V]
@@ -172,6 +175,18 @@ class Oracle:
         na = self.lib.orc_condition(ap.size - 1, ap.ctypes.data, aj.ctypes.data, out.ctypes.data, nagg, min_size,
                                     max_size)
         return out, int(na)
+
+    def flood_fill(self, ap, aj, k, starter):
+        """Flood-fill distance-k MIS (orc_flood_fill_mis) and aggregation to the
+        nearest root (orc_aggregate_to_nearest); returns (roots, agg, nagg)."""
+        ap = np.ascontiguousarray(ap, dtype=np.int32)
+        aj = np.ascontiguousarray(aj, dtype=np.int32)
+        n = ap.size - 1
+        roots = np.empty(n, dtype=np.int32)
+        agg = np.empty(n, dtype=np.int32)
+        self.lib.orc_flood_fill_mis(n, ap.ctypes.data, aj.ctypes.data, k, starter, roots.ctypes.data)
+        na = self.lib.orc_aggregate_to_nearest(n, ap.ctypes.data, aj.ctypes.data, roots.ctypes.data, agg.ctypes.data)
+        return roots, agg, int(na)
 
     def induced_graph(self, ap, aj, agg, nc):
         ap = np.ascontiguousarray(ap, dtype=np.int32)

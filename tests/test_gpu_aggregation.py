@@ -95,3 +95,48 @@ def test_aggmis_hierarchy(fem, gpu, oracle, meshgen, precision):
     report(f"      PCG: {info['iterations']} iterations, rel error {e:.1e}")
     assert info["status"] == 0 and e <= 1e-8
     s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("agg_type", [4, 5])
+def test_flood_fill_hierarchy(fem, gpu, oracle, meshgen, agg_type):
+    """aggregatorType 4 and 5 (randMisParameters 20027036: flood-fill MIS of
+    depth 2, sizes [27, 36]): the level-0 aggregates of the hierarchy equal the
+    oracle's flood-fill MIS and nearest-root aggregation from the start node the
+    library derives from the seed, merge-split conditioned for type 4 only, and
+    PCG on this hierarchy reaches the direct solution."""
+    import ctypes as C
+    import scipy.sparse.linalg as spla
+    report(f"[gpu] aggregatorType {agg_type} hierarchy")
+    xyz, tets = meshgen.cube_mesh(21, seed=3)
+    kappa = meshgen.lognormal_kappa(3, tets.shape[0], 1.0)
+    s = fem.FEMSolver.from_arrays(3, xyz, tets, kappa=kappa, mass_coeff=1e-2)
+    s.set_params(solverType=1, tolerance=1e-10, maxIters=300, precision=1, smootherType=1,
+                 preInnerIters=3, postInnerIters=3, topSize=64, aggregatorType=agg_type,
+                 randMisParameters=20027036, strengthThreshold=0.0, seed=0x5EED)
+    s.setup()
+    rp, ci, v, agg = s.level_matrix(0)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = ci != rows
+    ap = np.concatenate([[0], np.cumsum(np.add.reduceat(keep, rp[:-1]))]).astype(np.int32)
+    aj = ci[keep].astype(np.int32)
+    roots = np.empty(n, dtype=np.int32)
+    lagg = np.empty(n, dtype=np.int32)
+    start, na = C.c_int64(-1), C.c_int64(0)
+    assert fem.lib().fem_flood_fill_aggregate(n, ap.ctypes.data, aj.ctypes.data, 2, 0x5EED, C.byref(start),
+                                              roots.ctypes.data, lagg.ctypes.data, C.byref(na)) == 0
+    ro, want, nw = oracle.flood_fill(ap, aj, 2, start.value)
+    if agg_type == 4:
+        want, nw = oracle.condition(ap, aj, want, nw, 27, 36)
+    sizes = np.bincount(agg)
+    report(f"      level 0: {n} rows -> {sizes.size} aggregates of {sizes.min()}..{sizes.max()}, "
+           f"{s.num_levels()} levels")
+    assert sizes.size == nw and np.array_equal(agg, want)
+    b = meshgen.gaussian_rhs(3, n)
+    x, info = s.solve(b)
+    xd = spla.spsolve(s.get_matrix_scipy().tocsc(), b)
+    e = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    report(f"      PCG: {info['iterations']} iterations, rel error {e:.1e}")
+    assert info["status"] == 0 and e <= 1e-8
+    s.close()
