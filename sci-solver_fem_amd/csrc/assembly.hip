@@ -774,6 +774,14 @@ int check_mesh_device(int64_t nconn, const int32_t* conn, int64_t nv, int64_t ne
 
 void plan_assembly_slices(Handle& h);
 
+// radix-sort passes cover only the bits that keys below `bound` use (the
+// sort is stable, so the order is the same as over all 32 bits)
+static int key_bits(int64_t bound) {
+  int b = 1;
+  while (b < 32 && ((int64_t)1 << b) < bound) ++b;
+  return b;
+}
+
 void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const double* kappa) {
   hipStream_t s = h.stream;
   const int64_t nv = h.nv, ne = h.ne;
@@ -842,7 +850,7 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
                                                            (int4*)tmp.get(), key.get(), idx.get());
     HIP_LAUNCH_CHECK();
     h.eperm.resize(ne);
-    dev_sort_pairs_u32_i32(key.get(), key2.get(), idx.get(), h.eperm.get(), (size_t)ne, 32, s);
+    dev_sort_pairs_u32_i32(key.get(), key2.get(), idx.get(), h.eperm.get(), (size_t)ne, key_bits(nv), s);
     h.conn.resize(4 * ne);
     h.kappa.resize(ne);
     permute_elems_kernel<<<grid_for(ne, 256), 256, 0, s>>>(ne, h.eperm.get(), (const int4*)tmp.get(),
@@ -857,7 +865,7 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
     v2e_keys_kernel<<<grid_for(ne, 256), 256, 0, s>>>(ne, npe, (const int4*)h.conn.get(), key.get(), val.get());
     HIP_LAUNCH_CHECK();
     h.v2e.resize(m);
-    dev_sort_pairs_u32_i32(key.get(), key2.get(), val.get(), h.v2e.get(), (size_t)m, 32, s);
+    dev_sort_pairs_u32_i32(key.get(), key2.get(), val.get(), h.v2e.get(), (size_t)m, key_bits(nv), s);
     h.v2e_ptr.resize(nv + 1);
     lower_bound_kernel<<<grid_for(nv + 1, 256), 256, 0, s>>>(nv, key2.get(), m, h.v2e_ptr.get());
     HIP_LAUNCH_CHECK();
