@@ -169,31 +169,45 @@ template <bool kFill>
 __global__ void __launch_bounds__(kRowBlock) pattern_rows_kernel(
     int64_t nv, int npe, const int32_t* __restrict__ v2e_ptr, const int32_t* __restrict__ v2e,
     const int4* __restrict__ conn, int32_t* __restrict__ cnt, const int32_t* __restrict__ rp,
-    int32_t* __restrict__ ci, const int32_t* __restrict__ sptr = nullptr, int2* __restrict__ inc = nullptr) {
+    int32_t* __restrict__ ci, int32_t* __restrict__ stash, const int32_t* __restrict__ sptr = nullptr,
+    int2* __restrict__ inc = nullptr) {
   __shared__ int lds[kRowBlock * kRowCap];
   // XCD-contiguous rows: neighbouring rows gather the same elements through one L2
   int64_t row = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (row >= nv) return;
-  if (kFill && cnt[row] < 0) return;
+  // rows that overflowed kRowCap in the count pass (marked -1 there; the
+  // global-memory count then stores their length) are filled by the global path
+  if (kFill && (cnt[row] < 0 || cnt[row] > kRowCap)) return;
   // lane-interleaved rows (entry t of lane j at t * kRowBlock + j): the
   // lanes' searches and shifts address distinct banks
   const LdsRow buf{lds + threadIdx.x};
-  int len = 0;
-  len = sorted_insert(buf, len, kRowCap, (int)row);
+  // the count pass keeps each sorted row in a stash laid out like the LDS
+  // rows (entry t of row r at ((r / 64) * kRowCap + t) * 64 + r % 64, a
+  // wavefront's entry t one contiguous run), and the fill pass loads it
+  // instead of sorting again
+  int32_t* const srow = stash + (row >> 6) * (int64_t)(kRowCap * kRowBlock) + (row & 63);
   const int pb = v2e_ptr[row], pe = v2e_ptr[row + 1];
-  // the element codes and vertices of kIncChunk incidences are loaded
-  // together (independent loads), then inserted in incidence order
-  for (int p0 = pb; p0 < pe && len >= 0; p0 += kIncChunk) {
-    int4 cc[kIncChunk];
+  int len = 0;
+  if (kFill) {
+    len = cnt[row];
+    for (int t = 0; t < len; ++t) buf[t] = srow[t * kRowBlock];
+  } else {
+    len = sorted_insert(buf, len, kRowCap, (int)row);
+    // the element codes and vertices of kIncChunk incidences are loaded
+    // together (independent loads), then inserted in incidence order
+    for (int p0 = pb; p0 < pe && len >= 0; p0 += kIncChunk) {
+      int4 cc[kIncChunk];
 #pragma unroll
-    for (int u = 0; u < kIncChunk; ++u) cc[u] = p0 + u < pe ? conn[v2e[p0 + u] >> 2] : make_int4(0, 0, 0, 0);
-    for (int u = 0; u < kIncChunk && p0 + u < pe && len >= 0; ++u) {
-      const int v[4] = {cc[u].x, cc[u].y, cc[u].z, cc[u].w};
-      for (int a = 0; a < npe && len >= 0; ++a) len = sorted_insert(buf, len, kRowCap, v[a]);
+      for (int u = 0; u < kIncChunk; ++u) cc[u] = p0 + u < pe ? conn[v2e[p0 + u] >> 2] : make_int4(0, 0, 0, 0);
+      for (int u = 0; u < kIncChunk && p0 + u < pe && len >= 0; ++u) {
+        const int v[4] = {cc[u].x, cc[u].y, cc[u].z, cc[u].w};
+        for (int a = 0; a < npe && len >= 0; ++a) len = sorted_insert(buf, len, kRowCap, v[a]);
+      }
     }
   }
   if (!kFill) {
     cnt[row] = len;  // -1 marks overflow
+    for (int t = 0; t < len; ++t) srow[t * kRowBlock] = buf[t];
   } else {
     int32_t o = rp[row];
     for (int t = 0; t < len; ++t) ci[o + t] = buf[t];
@@ -873,8 +887,10 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
   // pattern
   {
     DevBuf<int32_t> cnt(nv);
+    // sorted rows of the count pass, kRowCap slots each (lane-interleaved per 64 rows)
+    DevBuf<int32_t> stash((size_t)((nv + kRowBlock - 1) / kRowBlock) * kRowBlock * kRowCap);
     pattern_rows_kernel<false><<<grid_for(nv, kRowBlock), kRowBlock, 0, s>>>(
-        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), nullptr, nullptr);
+        nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), nullptr, nullptr, stash.get());
     HIP_LAUNCH_CHECK();
     // long rows
     DevBuf<int32_t> long_rows(nv);
@@ -911,7 +927,7 @@ void build_mesh_device(Handle& h, const double* xyz, const int32_t* conn, const 
     if (with_inc) plan_assembly_slices(h);
     pattern_rows_kernel<true><<<grid_for(nv, kRowBlock), kRowBlock, 0, s>>>(
         nv, npe, h.v2e_ptr.get(), h.v2e.get(), (const int4*)h.conn.get(), cnt.get(), h.A.rp.get(), h.A.ci.get(),
-        with_inc ? h.asm_sptr.get() : nullptr, with_inc ? h.asm_inc.get() : nullptr);
+        stash.get(), with_inc ? h.asm_sptr.get() : nullptr, with_inc ? h.asm_inc.get() : nullptr);
     HIP_LAUNCH_CHECK();
     if (with_inc) h.asm_ready = true;
     if (nlong) {
