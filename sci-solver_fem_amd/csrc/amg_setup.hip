@@ -129,9 +129,17 @@ __global__ void roots_kernel(int64_t n, const int* __restrict__ mis, const int32
 // rule 0: aggregate of the first aggregated neighbour in adjacency order
 //         (what the reference's allocateNodesKernel computes);
 // rule 1: most frequent neighbouring aggregate, ties to the smallest id.
-__global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__ adj_ptr,
-                                const int32_t* __restrict__ adj, const int* __restrict__ part_in,
-                                int* __restrict__ part_out, int* __restrict__ aggregated) {
+// Rule 1 counts over the row's neighbour aggregates once per distinct one;
+// rows of at most kAllocCap neighbours read them from lane-interleaved LDS
+// (entry t of lane j at t * kAllocNT + j), longer rows from memory.
+constexpr int kAllocNT = 64;
+constexpr int kAllocCap = 32;
+__global__ void __launch_bounds__(kAllocNT) allocate_kernel(int64_t n, int rule, const int32_t* __restrict__ adj_ptr,
+                                                           const int32_t* __restrict__ adj,
+                                                           const int* __restrict__ part_in,
+                                                           int* __restrict__ part_out,
+                                                           int* __restrict__ aggregated) {
+  __shared__ int s_nb[kAllocCap * kAllocNT];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (aggregated[i] != 0) {
@@ -139,8 +147,9 @@ __global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__
     return;
   }
   int add_to = -1;
+  const int p0 = adj_ptr[i], p1 = adj_ptr[i + 1];
   if (rule == 0) {
-    for (int p = adj_ptr[i]; p < adj_ptr[i + 1]; ++p) {
+    for (int p = p0; p < p1; ++p) {
       int c = part_in[adj[p]];
       if (c != -1) {
         add_to = c;
@@ -148,20 +157,31 @@ __global__ void allocate_kernel(int64_t n, int rule, const int32_t* __restrict__
       }
     }
   } else {
-    int best_cnt = 0;
-    for (int p = adj_ptr[i]; p < adj_ptr[i + 1]; ++p) {
-      int c = part_in[adj[p]];
-      if (c == -1) continue;
-      bool seen = false;
-      for (int q = adj_ptr[i]; q < p; ++q)
-        if (part_in[adj[q]] == c) { seen = true; break; }
-      if (seen) continue;
-      int cnt = 0;
-      for (int q = p; q < adj_ptr[i + 1]; ++q) cnt += (part_in[adj[q]] == c);
-      if (cnt > best_cnt || (cnt == best_cnt && c < add_to)) {
-        best_cnt = cnt;
-        add_to = c;
+    // the row's neighbour aggregates, in adjacency order
+    auto count_best = [&](auto nb) {
+      const int deg = p1 - p0;
+      int best_cnt = 0;
+      for (int p = 0; p < deg; ++p) {
+        const int c = nb(p);
+        if (c == -1) continue;
+        bool seen = false;
+        for (int q = 0; q < p; ++q)
+          if (nb(q) == c) { seen = true; break; }
+        if (seen) continue;
+        int cnt = 0;
+        for (int q = p; q < deg; ++q) cnt += (nb(q) == c);
+        if (cnt > best_cnt || (cnt == best_cnt && c < add_to)) {
+          best_cnt = cnt;
+          add_to = c;
+        }
       }
+    };
+    if (p1 - p0 <= kAllocCap) {
+      int* const lb = s_nb + threadIdx.x;
+      for (int p = p0; p < p1; ++p) lb[(p - p0) * kAllocNT] = part_in[adj[p]];
+      count_best([lb](int t) { return lb[t * kAllocNT]; });
+    } else {
+      count_best([=](int t) { return part_in[adj[p0 + t]]; });
     }
   }
   part_out[i] = add_to;
@@ -763,8 +783,8 @@ void aggregate_graph(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int 
   unsigned prev_unalloc = 0xffffffffu;
   for (int loop = 0;; ++loop) {
     if (loop > 1000000) fail(FEM_ERR_NUMERIC, "aggregation did not complete");
-    allocate_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, rule, adj_ptr, adj, part_in.get(), part_out.get(),
-                                                     aggregated.get());
+    allocate_kernel<<<grid_for(n, kAllocNT), kAllocNT, 0, s>>>(n, rule, adj_ptr, adj, part_in.get(), part_out.get(),
+                                                               aggregated.get());
     HIP_LAUNCH_CHECK();
     std::swap(part_in, part_out);
     HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned), s));
